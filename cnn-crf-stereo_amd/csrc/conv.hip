@@ -1,0 +1,513 @@
+// Feature-CNN layers: fused valid convolution (kernel flipped, CUDNN_CONVOLUTION
+// semantics of stereo/operators/convolution.cpp:108,149) + bias + tanh.
+//
+// Feature maps are channel-last, H x W x Cs with Cs = C rounded up to 8, so the
+// im2col row of an output pixel is KH runs of KW * Cs contiguous elements and
+// every 8-element group (16 bytes in bf16) stays inside one tap.  The
+// reduction index is k = tap * Cs + c with tap = i * KW + j, and the packed
+// weight holds W[n][c][KH-1-i][KW-1-j] at k (the flip of true convolution).
+//
+// k_conv_bf16: implicit GEMM on v_mfma_f32_16x16x32_bf16.  A workgroup of 4
+//   waves computes a 4-row x 64-pixel output tile for all (<= 16*NT) output
+//   channels; the (4+KH-1) x (64+KW-1) input halo tile sits in LDS, packed
+//   weights stream through LDS in chunk groups; each wave owns one output row
+//   (4 M-tiles x NT N-tiles of accumulators).  Epilogue: + bias, tanh, bf16,
+//   staged through LDS for 208-byte-per-pixel coalesced stores.
+// k_conv_f32: the same on v_mfma_f32_16x16x4_f32 (2-row tile, float32 maps).
+// k_conv_generic: VALU direct convolution for the 1- and 3-channel input layer
+//   and any shape the MFMA kernels do not cover.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace ccs {
+
+// ---------------------------------------------------------------------------
+// Weight packing
+
+// bf16 layout: [q][n][RPB] with q = chunk of 32 reduction indices, n < 16*NT,
+// RPB = 40 elements per row (80 bytes: conflict-free 16-byte LDS reads).
+constexpr int RPB = 40;
+// f32 layout: [q][kk < 4][n] (n contiguous).
+
+__device__ __forceinline__ float flipped_weight(const float* __restrict__ w, int n, int c, int tap, int K, int C,
+                                                int KH, int KW) {
+  const int taps = KH * KW;
+  if (n >= K || c >= C || tap >= taps) return 0.f;
+  const int i = tap / KW, j = tap % KW;
+  return w[(((long)n * C + c) * KH + (KH - 1 - i)) * KW + (KW - 1 - j)];
+}
+
+__global__ void k_pack_bf16(const float* __restrict__ w, int K, int C, int Cs, int KH, int KW, int NQ, int NP,
+                            bf16* __restrict__ out) {
+  const long total = (long)NQ * NP * RPB;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int kk = (int)(idx % RPB);
+  const int n = (int)((idx / RPB) % NP);
+  const int q = (int)(idx / ((long)RPB * NP));
+  float v = 0.f;
+  if (kk < 32) {
+    const int k = q * 32 + kk;
+    const int tap = k / Cs, c = k % Cs;
+    v = flipped_weight(w, n, c, tap, K, C, KH, KW);
+  }
+  out[idx] = f32_to_bf16(v);
+}
+
+__global__ void k_pack_f32(const float* __restrict__ w, int K, int C, int Cs, int KH, int KW, int NQ, int NP,
+                           float* __restrict__ out) {
+  const long total = (long)NQ * 4 * NP;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int n = (int)(idx % NP);
+  const int kk = (int)((idx / NP) % 4);
+  const int q = (int)(idx / ((long)NP * 4));
+  const int k = q * 4 + kk;
+  out[idx] = flipped_weight(w, n, k % Cs, k / Cs, K, C, KH, KW);
+}
+
+// ---------------------------------------------------------------------------
+// BF16 MFMA implicit GEMM
+
+constexpr int kTH = 4;    // output rows per workgroup (one per wave)
+constexpr int kTW = 64;   // output pixels per row
+constexpr int kLdsBytes = 160 * 1024;
+
+template <int KH, int KW, int CG, int NT>
+struct Bf16Cfg {
+  static constexpr int Cs = CG * 8;
+  static constexpr int taps = KH * KW;
+  static constexpr int NQ = (taps * CG + 3) / 4;  // 32-wide reduction chunks
+  static constexpr int NP = 16 * NT;
+  static constexpr int AR = kTH + KH - 1, AC = kTW + KW - 1;
+  static constexpr int a_bytes = (AR * AC * Cs * 2 + 15) / 16 * 16;
+  static constexpr int chunk_bytes = NP * RPB * 2;
+  static constexpr int QG_max = (kLdsBytes - a_bytes) / chunk_bytes;
+  static constexpr int QG = QG_max < NQ ? QG_max : NQ;
+  static constexpr int out_bytes = kTH * kTW * Cs * 2;
+  static_assert(QG >= 1, "conv tile does not fit in LDS");
+  static_assert(out_bytes <= a_bytes, "output staging must fit in the input tile");
+};
+
+template <int KH, int KW, int CG, int NT>
+__global__ __launch_bounds__(256) void k_conv_bf16(const bf16* __restrict__ x, int Hin, int Win,
+                                                   const bf16* __restrict__ wpk, const float* __restrict__ bias,
+                                                   int K, int act, bf16* __restrict__ y, int Ho, int Wo) {
+  using Cfg = Bf16Cfg<KH, KW, CG, NT>;
+  constexpr int Cs = Cfg::Cs;
+  extern __shared__ __align__(16) unsigned char lds[];
+  bf16* As = reinterpret_cast<bf16*>(lds);
+  bf16* Bs = reinterpret_cast<bf16*>(lds + Cfg::a_bytes);
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int w0 = blockIdx.x * kTW, h0 = blockIdx.y * kTH;
+
+  // Stage the input halo tile: AR rows of AC pixels, 16-byte vectors (8 channels).
+  {
+    constexpr int vec_per_row = Cfg::AC * CG;
+    constexpr int total = Cfg::AR * vec_per_row;
+    for (int v = tid; v < total; v += 256) {
+      const int r = v / vec_per_row, rem = v % vec_per_row;
+      const int px = rem / CG, cg = rem % CG;
+      const int hi = h0 + r, wi = w0 + px;
+      uint4 val = make_uint4(0, 0, 0, 0);
+      if (hi < Hin && wi < Win)
+        val = *reinterpret_cast<const uint4*>(x + (((long)hi * Win + wi) * Cs + cg * 8));
+      *reinterpret_cast<uint4*>(As + ((r * Cfg::AC + px) * Cs + cg * 8)) = val;
+    }
+  }
+
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int lrow = lane & 15, lgrp = lane >> 4;
+  for (int q0 = 0; q0 < Cfg::NQ; q0 += Cfg::QG) {
+    const int nq = (Cfg::NQ - q0) < Cfg::QG ? (Cfg::NQ - q0) : Cfg::QG;
+    __syncthreads();  // previous group fully consumed (and A tile staged on the first pass)
+    {
+      const int nvec = nq * Cfg::chunk_bytes / 16;
+      const uint4* src = reinterpret_cast<const uint4*>(wpk + (long)q0 * Cfg::NP * RPB);
+      uint4* dst = reinterpret_cast<uint4*>(Bs);
+      for (int v = tid; v < nvec; v += 256) dst[v] = src[v];
+    }
+    __syncthreads();
+    for (int ql = 0; ql < nq; ++ql) {
+      const int kg = 4 * (q0 + ql) + lgrp;  // 8-element group of the reduction index
+      const int tap = kg / CG, c0 = (kg % CG) * 8;
+      const bool kvalid = tap < Cfg::taps;
+      const int ti = kvalid ? tap / KW : 0, tj = kvalid ? tap % KW : 0;
+      bf16x8 a[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        if (kvalid) {
+          const int px = 16 * m + lrow + tj;
+          a[m] = *reinterpret_cast<const bf16x8*>(As + (((wave + ti) * Cfg::AC + px) * Cs + c0));
+        } else {
+          a[m] = bf16x8{};
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + ((ql * Cfg::NP + 16 * t + lrow) * RPB + 8 * lgrp));
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][t], 0, 0, 0);
+      }
+    }
+  }
+
+  // Epilogue: bias + tanh -> bf16 into LDS [row][pixel][Cs], then coalesced stores.
+  __syncthreads();
+  bf16* Os = As;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int n = 16 * t + lrow;
+    const float bn = (n < K && bias) ? bias[n] : 0.f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int px = 16 * m + 4 * lgrp + e;
+        float v = acc[m][t][e] + bn;
+        if (act) v = tanhf(v);
+        if (n < Cs) Os[(wave * kTW + px) * Cs + n] = f32_to_bf16(n < K ? v : 0.f);
+      }
+  }
+  __syncthreads();
+  {
+    constexpr int vec_per_row = kTW * CG;
+    for (int v = tid; v < kTH * vec_per_row; v += 256) {
+      const int r = v / vec_per_row, rem = v % vec_per_row;
+      const int px = rem / CG, cg = rem % CG;
+      const int ho = h0 + r, wo = w0 + px;
+      if (ho < Ho && wo < Wo)
+        *reinterpret_cast<uint4*>(y + (((long)ho * Wo + wo) * Cs + cg * 8)) =
+            *reinterpret_cast<const uint4*>(Os + ((r * kTW + px) * Cs + cg * 8));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 MFMA implicit GEMM (v_mfma_f32_16x16x4_f32)
+
+constexpr int kTHf = 2;
+constexpr int kTWf = 64;
+
+template <int KH, int KW, int CG, int NT>
+struct F32Cfg {
+  static constexpr int Cs = CG * 8;
+  static constexpr int CP = Cs + 1;  // LDS pixel pitch in floats (odd: conflict-free)
+  static constexpr int taps = KH * KW;
+  static constexpr int NQ = (taps * Cs + 3) / 4;
+  static constexpr int NP = 16 * NT;
+  static constexpr int AR = kTHf + KH - 1, AC = kTWf + KW - 1;
+  static constexpr int a_bytes = (AR * AC * CP * 4 + 15) / 16 * 16;
+  static constexpr int chunk_bytes = 4 * NP * 4;
+  static constexpr int QG_max = (kLdsBytes - a_bytes) / chunk_bytes;
+  static constexpr int QG = QG_max < NQ ? QG_max : NQ;
+  static constexpr int out_bytes = kTHf * kTWf * Cs * 4;
+  static_assert(QG >= 1, "conv tile does not fit in LDS");
+  static_assert(out_bytes <= a_bytes, "output staging must fit in the input tile");
+};
+
+template <int KH, int KW, int CG, int NT>
+__global__ __launch_bounds__(256) void k_conv_f32(const float* __restrict__ x, int Hin, int Win,
+                                                  const float* __restrict__ wpk, const float* __restrict__ bias,
+                                                  int K, int act, float* __restrict__ y, int Ho, int Wo) {
+  using Cfg = F32Cfg<KH, KW, CG, NT>;
+  constexpr int Cs = Cfg::Cs, CP = Cfg::CP;
+  extern __shared__ __align__(16) unsigned char lds[];
+  float* As = reinterpret_cast<float*>(lds);
+  float* Bs = reinterpret_cast<float*>(lds + Cfg::a_bytes);
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wrow = wave >> 1, wcol = (wave & 1) * 32;
+  const int w0 = blockIdx.x * kTWf, h0 = blockIdx.y * kTHf;
+  {
+    constexpr int per_row = Cfg::AC * Cs;
+    constexpr int total = Cfg::AR * per_row;
+    for (int v = tid; v < total; v += 256) {
+      const int r = v / per_row, rem = v % per_row;
+      const int px = rem / Cs, c = rem % Cs;
+      const int hi = h0 + r, wi = w0 + px;
+      float val = 0.f;
+      if (hi < Hin && wi < Win) val = x[((long)hi * Win + wi) * Cs + c];
+      As[(r * Cfg::AC + px) * CP + c] = val;
+    }
+  }
+  f32x4 acc[2][NT];
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int lrow = lane & 15, lk = lane >> 4;
+  for (int q0 = 0; q0 < Cfg::NQ; q0 += Cfg::QG) {
+    const int nq = (Cfg::NQ - q0) < Cfg::QG ? (Cfg::NQ - q0) : Cfg::QG;
+    __syncthreads();
+    {
+      const int nvec = nq * Cfg::chunk_bytes / 16;
+      const float4* src = reinterpret_cast<const float4*>(wpk + (long)q0 * 4 * Cfg::NP);
+      float4* dst = reinterpret_cast<float4*>(Bs);
+      for (int v = tid; v < nvec; v += 256) dst[v] = src[v];
+    }
+    __syncthreads();
+    for (int ql = 0; ql < nq; ++ql) {
+      const int k = 4 * (q0 + ql) + lk;
+      const int tap = k / Cs, c = k % Cs;
+      const bool kvalid = tap < Cfg::taps;
+      const int ti = kvalid ? tap / KW : 0, tj = kvalid ? tap % KW : 0;
+      float a[2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+        a[m] = kvalid ? As[((wrow + ti) * Cfg::AC + wcol + 16 * m + lrow + tj) * CP + c] : 0.f;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const float b = Bs[(ql * 4 + lk) * Cfg::NP + 16 * t + lrow];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], b, acc[m][t], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();
+  float* Os = As;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int n = 16 * t + lrow;
+    const float bn = (n < K && bias) ? bias[n] : 0.f;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int px = wcol + 16 * m + 4 * (lane >> 4) + e;
+        float v = acc[m][t][e] + bn;
+        if (act) v = tanhf(v);
+        if (n < Cs) Os[(wrow * kTWf + px) * Cs + n] = n < K ? v : 0.f;
+      }
+  }
+  __syncthreads();
+  {
+    constexpr int vec_per_row = kTWf * Cs / 4;
+    for (int v = tid; v < kTHf * vec_per_row; v += 256) {
+      const int r = v / vec_per_row, rem = v % vec_per_row;
+      const int px = (rem * 4) / Cs;
+      const int ho = h0 + r, wo = w0 + px;
+      if (ho < Ho && wo < Wo)
+        *reinterpret_cast<float4*>(y + ((long)ho * Wo + w0) * Cs + (long)rem * 4) =
+            *reinterpret_cast<const float4*>(Os + (r * kTWf) * Cs + rem * 4);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generic VALU convolution: one thread per (output pixel, 8 output channels).
+// Input either channel-last (stride Cs_in, bf16 or f32) or C x H x W float32.
+
+template <typename TI, typename TO>
+__global__ void k_conv_generic(const TI* __restrict__ x, int x_nchw, int Hin, int Win, int C, int Cs_in,
+                               const float* __restrict__ w, const float* __restrict__ bias, int K, int KH, int KW,
+                               int act, TO* __restrict__ y, int Ho, int Wo, int Cs_out) {
+  const int groups = Cs_out / 8;
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)Ho * Wo * groups) return;
+  const int g = (int)(gid % groups);
+  const long p = gid / groups;
+  const int ho = (int)(p / Wo), wo = (int)(p % Wo);
+  float acc[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int n = g * 8 + e;
+    acc[e] = (n < K && bias) ? bias[n] : 0.f;
+  }
+  for (int i = 0; i < KH; ++i)
+    for (int j = 0; j < KW; ++j)
+      for (int c = 0; c < C; ++c) {
+        const long xi = x_nchw ? ((long)c * Hin + ho + i) * Win + wo + j
+                               : ((long)(ho + i) * Win + wo + j) * Cs_in + c;
+        const float xv = (float)x[xi];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int n = g * 8 + e;
+          if (n < K) acc[e] += xv * w[(((long)n * C + c) * KH + (KH - 1 - i)) * KW + (KW - 1 - j)];
+        }
+      }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int n = g * 8 + e;
+    float v = acc[e];
+    if (act) v = tanhf(v);
+    y[p * Cs_out + n] = (TO)(n < K ? v : 0.f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch
+
+struct ConvShape {
+  int H, W, C, K, KH, KW;
+};
+
+template <int KH, int KW, int CG, int NT>
+static bool try_bf16(const ConvShape& s, const void* x, const void* wpk, const float* b, int act, void* y,
+                     hipStream_t st) {
+  if (s.KH != KH || s.KW != KW || feature_stride(s.C) != CG * 8 || (s.K + 15) / 16 != NT) return false;
+  using Cfg = Bf16Cfg<KH, KW, CG, NT>;
+  const int Ho = s.H - KH + 1, Wo = s.W - KW + 1;
+  const size_t lds = Cfg::a_bytes + (size_t)Cfg::QG * Cfg::chunk_bytes;
+  static bool configured = false;
+  if (!configured) {
+    hipFuncSetAttribute((const void*)k_conv_bf16<KH, KW, CG, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        kLdsBytes);
+    configured = true;
+  }
+  k_conv_bf16<KH, KW, CG, NT><<<dim3(ceil_div(Wo, kTW), ceil_div(Ho, kTH)), 256, lds, st>>>(
+      (const bf16*)x, s.H, s.W, (const bf16*)wpk, b, s.K, act, (bf16*)y, Ho, Wo);
+  return true;
+}
+
+template <int KH, int KW, int CG, int NT>
+static bool try_f32(const ConvShape& s, const void* x, const void* wpk, const float* b, int act, void* y,
+                    hipStream_t st) {
+  if (s.KH != KH || s.KW != KW || feature_stride(s.C) != CG * 8 || (s.K + 15) / 16 != NT) return false;
+  using Cfg = F32Cfg<KH, KW, CG, NT>;
+  const int Ho = s.H - KH + 1, Wo = s.W - KW + 1;
+  const size_t lds = Cfg::a_bytes + (size_t)Cfg::QG * Cfg::chunk_bytes;
+  static bool configured = false;
+  if (!configured) {
+    hipFuncSetAttribute((const void*)k_conv_f32<KH, KW, CG, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        kLdsBytes);
+    configured = true;
+  }
+  k_conv_f32<KH, KW, CG, NT><<<dim3(ceil_div(Wo, kTWf), ceil_div(Ho, kTHf)), 256, lds, st>>>(
+      (const float*)x, s.H, s.W, (const float*)wpk, b, s.K, act, (float*)y, Ho, Wo);
+  return true;
+}
+
+// MFMA kernels exist for the 100-channel layers of both tower architectures
+// (3x3 and 2x2) and for 8-channel layers (small test networks).  They write the
+// output with the input's channel stride, so both strides must agree.
+static bool mfma_supported(const ConvShape& s) {
+  const int cs = feature_stride(s.C), nt = (s.K + 15) / 16;
+  const bool kern = (s.KH == 3 && s.KW == 3) || (s.KH == 2 && s.KW == 2);
+  return kern && feature_stride(s.K) == cs && ((cs == 104 && nt == 7) || (cs == 8 && nt == 1));
+}
+
+static bool launch_mfma(const ConvShape& s, int precision, const void* x, const void* wpk, const float* b,
+                        int act, void* y, hipStream_t st) {
+  if (precision == CCS_BF16) {
+    return try_bf16<3, 3, 13, 7>(s, x, wpk, b, act, y, st) || try_bf16<2, 2, 13, 7>(s, x, wpk, b, act, y, st) ||
+           try_bf16<3, 3, 1, 1>(s, x, wpk, b, act, y, st) || try_bf16<2, 2, 1, 1>(s, x, wpk, b, act, y, st);
+  }
+  return try_f32<3, 3, 13, 7>(s, x, wpk, b, act, y, st) || try_f32<2, 2, 13, 7>(s, x, wpk, b, act, y, st) ||
+         try_f32<3, 3, 1, 1>(s, x, wpk, b, act, y, st) || try_f32<2, 2, 1, 1>(s, x, wpk, b, act, y, st);
+}
+
+static size_t packed_bytes(int K, int C, int KH, int KW, int precision) {
+  const int Cs = feature_stride(C);
+  const int NP = 16 * ((K + 15) / 16);
+  if (precision == CCS_BF16) {
+    const int NQ = (KH * KW * Cs / 8 + 3) / 4;
+    return (size_t)NQ * NP * RPB * 2;
+  }
+  const int NQ = (KH * KW * Cs + 3) / 4;
+  return (size_t)NQ * 4 * NP * 4;
+}
+
+}  // namespace ccs
+
+using namespace ccs;
+
+extern "C" size_t ccs_conv_packed_bytes(int K, int C, int KH, int KW, int precision) {
+  return packed_bytes(K, C, KH, KW, precision);
+}
+
+extern "C" int ccs_conv_pack_weights(ccs_handle_t h, const float* w, int K, int C, int KH, int KW, int precision,
+                                     void* packed) {
+  CCS_CHECK(h && w && packed, CCS_ERR_ARG, "ccs_conv_pack_weights: NULL argument");
+  CCS_CHECK(K >= 1 && C >= 1 && KH >= 1 && KW >= 1, CCS_ERR_ARG, "ccs_conv_pack_weights: bad shape");
+  CCS_CHECK(precision == CCS_FP32 || precision == CCS_BF16, CCS_ERR_ARG, "ccs_conv_pack_weights: bad precision");
+  const int Cs = feature_stride(C);
+  const int NP = 16 * ((K + 15) / 16);
+  if (precision == CCS_BF16) {
+    const int NQ = (KH * KW * Cs / 8 + 3) / 4;
+    const long total = (long)NQ * NP * RPB;
+    k_pack_bf16<<<ceil_div(total, 256), 256, 0, h->stream>>>(w, K, C, Cs, KH, KW, NQ, NP, (bf16*)packed);
+  } else {
+    const int NQ = (KH * KW * Cs + 3) / 4;
+    const long total = (long)NQ * 4 * NP;
+    k_pack_f32<<<ceil_div(total, 256), 256, 0, h->stream>>>(w, K, C, Cs, KH, KW, NQ, NP, (float*)packed);
+  }
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+static int conv_generic(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C, const float* w,
+                        const float* b, int K, int KH, int KW, int act, int precision, void* y) {
+  const int Ho = H - KH + 1, Wo = W - KW + 1;
+  const int Cs_out = feature_stride(K);
+  const long threads = (long)Ho * Wo * (Cs_out / 8);
+  const unsigned grid = ceil_div(threads, 256);
+  const int nchw = x_layout == CCS_LAYOUT_NCHW;
+  const int Cs_in = nchw ? C : feature_stride(C);
+  if (precision == CCS_BF16) {
+    if (nchw)
+      k_conv_generic<float, bf16><<<grid, 256, 0, h->stream>>>((const float*)x, 1, H, W, C, Cs_in, w, b, K, KH, KW,
+                                                              act, (bf16*)y, Ho, Wo, Cs_out);
+    else
+      k_conv_generic<bf16, bf16><<<grid, 256, 0, h->stream>>>((const bf16*)x, 0, H, W, C, Cs_in, w, b, K, KH, KW,
+                                                             act, (bf16*)y, Ho, Wo, Cs_out);
+  } else {
+    k_conv_generic<float, float><<<grid, 256, 0, h->stream>>>((const float*)x, nchw, H, W, C, Cs_in, w, b, K, KH,
+                                                             KW, act, (float*)y, Ho, Wo, Cs_out);
+  }
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+static int check_conv(const void* x, int x_layout, int H, int W, int C, int K, int KH, int KW, int precision,
+                      const void* y) {
+  CCS_CHECK(x && y, CCS_ERR_ARG, "ccs_conv2d: NULL tensor");
+  CCS_CHECK(precision == CCS_FP32 || precision == CCS_BF16, CCS_ERR_ARG, "ccs_conv2d: bad precision %d", precision);
+  CCS_CHECK(x_layout == CCS_LAYOUT_NHWC || x_layout == CCS_LAYOUT_NCHW, CCS_ERR_ARG, "ccs_conv2d: bad layout");
+  CCS_CHECK(C >= 1 && K >= 1 && KH >= 1 && KW >= 1 && H >= KH && W >= KW, CCS_ERR_SHAPE,
+            "ccs_conv2d: input %dx%d smaller than the %dx%d kernel (or empty)", H, W, KH, KW);
+  CCS_CHECK(C <= 4096 && K <= 4096, CCS_ERR_UNSUPPORTED, "ccs_conv2d: too many channels");
+  return CCS_OK;
+}
+
+extern "C" int ccs_conv_forward_packed(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C,
+                                       const void* packed, const float* b, int K, int KH, int KW, int act,
+                                       int precision, void* y) {
+  CCS_CHECK(h && packed, CCS_ERR_ARG, "ccs_conv_forward_packed: NULL argument");
+  int rc = check_conv(x, x_layout, H, W, C, K, KH, KW, precision, y);
+  if (rc) return rc;
+  const ConvShape s{H, W, C, K, KH, KW};
+  CCS_CHECK(x_layout == CCS_LAYOUT_NHWC && mfma_supported(s), CCS_ERR_UNSUPPORTED,
+            "ccs_conv_forward_packed: no MFMA kernel for C=%d K=%d %dx%d (use ccs_conv2d_forward)", C, K, KH, KW);
+  launch_mfma(s, precision, x, packed, b, act, y, h->stream);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_conv2d_forward(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C, const float* w,
+                                  const float* b, int K, int KH, int KW, int act, int precision, void* y) {
+  CCS_CHECK(h && w, CCS_ERR_ARG, "ccs_conv2d_forward: NULL argument");
+  int rc = check_conv(x, x_layout, H, W, C, K, KH, KW, precision, y);
+  if (rc) return rc;
+  const ConvShape s{H, W, C, K, KH, KW};
+  if (x_layout == CCS_LAYOUT_NHWC && mfma_supported(s)) {
+    if ((rc = h->conv_w.reserve(packed_bytes(K, C, KH, KW, precision)))) return rc;
+    if ((rc = ccs_conv_pack_weights(h, w, K, C, KH, KW, precision, h->conv_w.ptr))) return rc;
+    launch_mfma(s, precision, x, h->conv_w.ptr, b, act, y, h->stream);
+    CCS_LAUNCH_CHECK();
+    return CCS_OK;
+  }
+  CCS_CHECK(x_layout == CCS_LAYOUT_NHWC || C <= 8, CCS_ERR_UNSUPPORTED,
+            "ccs_conv2d_forward: NCHW input is only accepted for image layers (C <= 8)");
+  return conv_generic(h, x, x_layout, H, W, C, w, b, K, KH, KW, act, precision, y);
+}

@@ -1,0 +1,445 @@
+// Correlation cost volume (stereo/operators/stereocorrelation.cu:111-142) with
+// the -softmax over disparities (stereo/operators/softmax.cpp:30-38,
+// stereo/utils.cu:289-292) fused into its epilogue, and the adjoints.
+//
+//   S[h][w][k] = sum_c f0[h][w][c] f1[h][w + o_k][c],  o_k = -(disp_min + k disp_step),
+//   0 where w + o_k leaves [0, W).
+//
+// Fast path k_corr_volume: one workgroup per (image row, 16*NW pixels).  Both
+// feature rows are staged in LDS (f1 as the window of columns the band needs,
+// zero outside the image), each wave computes the 16 x (16*NTM) block
+// S[w][w'] of its 16 pixels against the window with MFMA (a banded GEMM over
+// the channels), parks it in LDS, and the epilogue gathers the band
+// w' = w + o_k, applies -softmax over k and writes the H x W x D volume with
+// 16-byte stores, D contiguous floats per pixel.
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+namespace ccs {
+
+template <typename T>
+struct CorrTraits;
+template <>
+struct CorrTraits<bf16> {
+  static constexpr int kChunk = 32;  // channels per MFMA
+};
+template <>
+struct CorrTraits<float> {
+  static constexpr int kChunk = 4;
+};
+
+struct CorrArgs {
+  const void* f0;
+  const void* f1;
+  float* out;
+  int H, W, Cs, D;
+  int o0, ostep;  // o_k = o0 + k * ostep
+  int o_lo;       // min_k o_k
+  int epilogue;
+  int pitch;      // LDS pixel pitch (elements)
+  int win;        // f1 window columns
+  int sp;         // S row pitch (floats)
+};
+
+template <typename T, int NTM>
+__global__ __launch_bounds__(256) void k_corr_volume(CorrArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int NW = blockDim.x >> 6;
+  const int TW = 16 * NW;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lrow = lane & 15, lgrp = lane >> 4;
+  const int h = blockIdx.y;
+  const int w0 = blockIdx.x * TW;
+  const int Cs = a.Cs, P = a.pitch;
+  T* F0 = reinterpret_cast<T*>(lds);
+  T* F1 = F0 + TW * P;
+  float* S = reinterpret_cast<float*>(F1 + a.win * P);
+  float* S_w = S + wave * 16 * a.sp;
+  float* stats = S + NW * 16 * a.sp;  // [NW][16][2]
+
+  const T* f0 = reinterpret_cast<const T*>(a.f0) + (long)h * a.W * Cs;
+  const T* f1 = reinterpret_cast<const T*>(a.f1) + (long)h * a.W * Cs;
+  const int wbase = w0 + a.o_lo;  // first f1 column of the window
+  if constexpr (sizeof(T) == 2) {
+    const int cg = Cs / 8;
+    for (int v = tid; v < TW * cg; v += blockDim.x) {
+      const int px = v / cg, g = v % cg, wi = w0 + px;
+      uint4 val = make_uint4(0, 0, 0, 0);
+      if (wi < a.W) val = *reinterpret_cast<const uint4*>(f0 + (long)wi * Cs + 8 * g);
+      *reinterpret_cast<uint4*>(F0 + px * P + 8 * g) = val;
+    }
+    for (int v = tid; v < a.win * cg; v += blockDim.x) {
+      const int px = v / cg, g = v % cg, wi = wbase + px;
+      uint4 val = make_uint4(0, 0, 0, 0);
+      if (wi >= 0 && wi < a.W) val = *reinterpret_cast<const uint4*>(f1 + (long)wi * Cs + 8 * g);
+      *reinterpret_cast<uint4*>(F1 + px * P + 8 * g) = val;
+    }
+  } else {
+    for (int v = tid; v < TW * Cs; v += blockDim.x) {
+      const int px = v / Cs, c = v % Cs, wi = w0 + px;
+      F0[px * P + c] = wi < a.W ? f0[(long)wi * Cs + c] : 0.f;
+    }
+    for (int v = tid; v < a.win * Cs; v += blockDim.x) {
+      const int px = v / Cs, c = v % Cs, wi = wbase + px;
+      F1[px * P + c] = (wi >= 0 && wi < a.W) ? f1[(long)wi * Cs + c] : 0.f;
+    }
+  }
+  __syncthreads();
+
+  f32x4 acc[NTM];
+#pragma unroll
+  for (int t = 0; t < NTM; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int arow = 16 * wave + lrow;  // pixel of this lane's A row
+  if constexpr (sizeof(T) == 2) {
+    const int cg = Cs / 8;
+    for (int q = 0; q < (cg + 3) / 4; ++q) {
+      const int g = 4 * q + lgrp;
+      const bool ok = g < cg;
+      const bf16x8 av = ok ? *reinterpret_cast<const bf16x8*>(F0 + arow * P + 8 * g) : bf16x8{};
+#pragma unroll
+      for (int t = 0; t < NTM; ++t) {
+        const int col = 16 * wave + 16 * t + lrow;
+        const bf16x8 bv = ok ? *reinterpret_cast<const bf16x8*>(F1 + col * P + 8 * g) : bf16x8{};
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+      }
+    }
+  } else {
+    for (int q = 0; q < Cs / 4; ++q) {
+      const int c = 4 * q + lgrp;
+      const float av = F0[arow * P + c];
+#pragma unroll
+      for (int t = 0; t < NTM; ++t) {
+        const int col = 16 * wave + 16 * t + lrow;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, F1[col * P + c], acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // S_w[r][j] = f0[pixel 16*wave + r] . f1[window column 16*wave + j]
+#pragma unroll
+  for (int t = 0; t < NTM; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) S_w[(4 * lgrp + e) * a.sp + 16 * t + lrow] = acc[t][e];
+  __syncthreads();
+
+  // Band entry of pixel r, disparity k: window column r + o_k - o_lo.
+  const int D = a.D;
+  const bool neg_softmax = a.epilogue == CCS_EPILOGUE_NEG_SOFTMAX;
+  if (neg_softmax) {
+    const int r = lane >> 2, s4 = lane & 3;
+    float mx = -INFINITY;
+    for (int k = s4; k < D; k += 4) mx = fmaxf(mx, S_w[r * a.sp + r + a.o0 + k * a.ostep - a.o_lo]);
+    mx = fmaxf(mx, __shfl_xor(mx, 1, 64));
+    mx = fmaxf(mx, __shfl_xor(mx, 2, 64));
+    float sum = 0.f;
+    for (int k = s4; k < D; k += 4) sum += expf(S_w[r * a.sp + r + a.o0 + k * a.ostep - a.o_lo] - mx);
+    sum += __shfl_xor(sum, 1, 64);
+    sum += __shfl_xor(sum, 2, 64);
+    if (s4 == 0) {
+      stats[(wave * 16 + r) * 2] = mx;
+      stats[(wave * 16 + r) * 2 + 1] = 1.f / sum;
+    }
+  }
+  __syncthreads();
+  const long pix0 = (long)h * a.W + w0 + 16 * wave;
+  const int npx = min(16, a.W - (w0 + 16 * wave));
+  if (npx <= 0) return;
+  float* outp = a.out + pix0 * D;
+  const int total = npx * D;
+  auto value = [&](int f) {
+    const int r = f / D, k = f % D;
+    const float s = S_w[r * a.sp + r + a.o0 + k * a.ostep - a.o_lo];
+    if (!neg_softmax) return s;
+    return -expf(s - stats[(wave * 16 + r) * 2]) * stats[(wave * 16 + r) * 2 + 1];
+  };
+  if ((D & 3) == 0) {
+    for (int f = 4 * lane; f < total; f += 256) {
+      float4 v;
+      v.x = value(f);
+      v.y = value(f + 1);
+      v.z = value(f + 2);
+      v.w = value(f + 3);
+      *reinterpret_cast<float4*>(outp + f) = v;
+    }
+  } else {
+    for (int f = lane; f < total; f += 64) outp[f] = value(f);
+  }
+}
+
+// Reference-compatible correlation on N x C x H x W float32 inputs (any offsets,
+// rect_corr); output NHWC or NCHW.  One thread per output element.
+__global__ void k_corr_generic(const float* __restrict__ im0, const float* __restrict__ im1,
+                               const int* __restrict__ disp, float* __restrict__ out, int N, int C, int H, int W,
+                               int D, int nhwc, int rc) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)N * H * W * D;
+  if (idx >= total) return;
+  int n, h, w, k;
+  if (nhwc) {
+    k = (int)(idx % D);
+    w = (int)((idx / D) % W);
+    h = (int)((idx / ((long)D * W)) % H);
+    n = (int)(idx / ((long)D * W * H));
+  } else {
+    w = (int)(idx % W);
+    h = (int)((idx / W) % H);
+    k = (int)((idx / ((long)W * H)) % D);
+    n = (int)(idx / ((long)W * H * D));
+  }
+  const long HW = (long)H * W;
+  const int w2 = w + disp[k];
+  float best = -100000.f;
+  for (int yy = -rc; yy <= rc; ++yy) {
+    const int h2 = h + yy;
+    if (h2 < 0 || h2 >= H) continue;
+    float r = 0.f;
+    if (w2 >= 0 && w2 < W)
+      for (int c = 0; c < C; ++c)
+        r += im0[((long)n * C + c) * HW + (long)h * W + w] * im1[((long)n * C + c) * HW + (long)h2 * W + w2];
+    if (r > best) best = r;
+  }
+  out[idx] = best;
+}
+
+// Reference-compatible adjoint (stereo/operators/stereocorrelation.cu:243-278):
+// one thread per (n, c, h, w), gather over disparities, no atomics.
+__global__ void k_corr_bwd_generic(float* __restrict__ d0, float* __restrict__ d1, const float* __restrict__ G,
+                                   const float* __restrict__ im0, const float* __restrict__ im1,
+                                   const int* __restrict__ disp, int N, int C, int H, int W, int D, int nhwc) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long HW = (long)H * W;
+  if (idx >= (long)N * C * HW) return;
+  const int w = (int)(idx % W);
+  const int h = (int)((idx / W) % H);
+  const int c = (int)((idx / HW) % C);
+  const int n = (int)(idx / (HW * C));
+  auto g_at = [&](int hh, int ww, int k) {
+    return nhwc ? G[(((long)n * H + hh) * W + ww) * D + k] : G[(((long)n * D + k) * H + hh) * W + ww];
+  };
+  float g0 = 0.f, g1 = 0.f;
+  for (int k = 0; k < D; ++k) {
+    const int wp = w + disp[k];
+    if (wp >= 0 && wp < W) g0 += im1[((long)n * C + c) * HW + (long)h * W + wp] * g_at(h, w, k);
+    const int wm = w - disp[k];
+    if (wm >= 0 && wm < W) g1 += im0[((long)n * C + c) * HW + (long)h * W + wm] * g_at(h, wm, k);
+  }
+  d0[idx] = g0;
+  d1[idx] = g1;
+}
+
+// Adjoint of the fast path (float32 channel-last maps): one thread per (h, w, c).
+__global__ void k_corr_volume_bwd(const float* __restrict__ f0, const float* __restrict__ f1, int H, int W, int C,
+                                  int Cs, int o0, int ostep, int D, const float* __restrict__ G,
+                                  float* __restrict__ df0, float* __restrict__ df1) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)H * W * Cs) return;
+  const int c = (int)(idx % Cs);
+  const long p = idx / Cs;
+  const int w = (int)(p % W), h = (int)(p / W);
+  float g0 = 0.f, g1 = 0.f;
+  if (c < C) {
+    const long row = (long)h * W;
+    for (int k = 0; k < D; ++k) {
+      const int o = o0 + k * ostep;
+      const int wp = w + o;
+      if (wp >= 0 && wp < W) g0 += f1[(row + wp) * Cs + c] * G[(row + w) * D + k];
+      const int wm = w - o;
+      if (wm >= 0 && wm < W) g1 += f0[(row + wm) * Cs + c] * G[(row + wm) * D + k];
+    }
+  }
+  df0[idx] = g0;
+  df1[idx] = g1;
+}
+
+// Cross entropy of softmax_D(S) at the target label; dS = softmax - onehot.
+__global__ void k_softmax_xent(const float* __restrict__ S, long HW, int D, const int* __restrict__ target,
+                               float* __restrict__ dS, double* __restrict__ part) {
+  double loss = 0.0;
+  const int sub = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (long p = (long)blockIdx.x * (blockDim.x >> 6) + wv; p < HW; p += (long)gridDim.x * (blockDim.x >> 6)) {
+    const float* s = S + p * D;
+    float mx = -INFINITY;
+    for (int k = sub; k < D; k += 64) mx = fmaxf(mx, s[k]);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    float sum = 0.f;
+    for (int k = sub; k < D; k += 64) sum += expf(s[k] - mx);
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    const float lse = mx + logf(sum);
+    const int t = target[p] < D ? target[p] : -1;  // labels outside [0, D) are ignored
+    for (int k = sub; k < D; k += 64) dS[p * D + k] = t >= 0 ? expf(s[k] - lse) - (k == t ? 1.f : 0.f) : 0.f;
+    if (sub == 0 && t >= 0) loss += (double)(lse - s[t]);
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = loss;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_f64_corr(const double* __restrict__ in, int n, double* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += in[i];
+    out[0] = s;
+  }
+}
+
+template <typename T, int NTM>
+static int launch_volume(CorrArgs a, int NW, hipStream_t st) {
+  const size_t bytes = (size_t)(16 * NW + a.win) * a.pitch * sizeof(T) + (size_t)NW * 16 * a.sp * 4 +
+                       (size_t)NW * 16 * 2 * 4;
+  static bool configured = false;
+  if (!configured) {
+    hipFuncSetAttribute((const void*)k_corr_volume<T, NTM>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    configured = true;
+  }
+  k_corr_volume<T, NTM><<<dim3(ceil_div(a.W, 16 * NW), a.H), 64 * NW, bytes, st>>>(a);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+}  // namespace ccs
+
+using namespace ccs;
+
+extern "C" int ccs_correlation_volume(ccs_handle_t h, const void* f0, const void* f1, int precision, int H, int W,
+                                      int C, int disp_min, int disp_step, int D, int epilogue, float* out) {
+  CCS_CHECK(h && f0 && f1 && out, CCS_ERR_ARG, "ccs_correlation_volume: NULL argument");
+  CCS_CHECK(H >= 1 && W >= 1 && C >= 1 && D >= 1, CCS_ERR_ARG, "ccs_correlation_volume: empty shape");
+  CCS_CHECK(precision == CCS_FP32 || precision == CCS_BF16, CCS_ERR_ARG, "ccs_correlation_volume: bad precision");
+  CCS_CHECK(epilogue == CCS_EPILOGUE_RAW || epilogue == CCS_EPILOGUE_NEG_SOFTMAX, CCS_ERR_ARG,
+            "ccs_correlation_volume: bad epilogue");
+  CCS_CHECK(disp_step != 0 || D == 1, CCS_ERR_ARG, "ccs_correlation_volume: zero disparity step");
+  CCS_CHECK(C <= 256, CCS_ERR_UNSUPPORTED, "ccs_correlation_volume: more than 256 channels");
+  CorrArgs a{};
+  a.f0 = f0; a.f1 = f1; a.out = out; a.H = H; a.W = W; a.Cs = feature_stride(C); a.D = D;
+  a.o0 = -disp_min; a.ostep = -disp_step; a.epilogue = epilogue;
+  const int olast = a.o0 + (D - 1) * a.ostep;
+  a.o_lo = a.o0 < olast ? a.o0 : olast;
+  const int span = (D - 1) * (disp_step < 0 ? -disp_step : disp_step);
+  const int ntm_need = (16 + span + 15) / 16;
+  int ntm;
+  if (ntm_need <= 3) ntm = 3;
+  else if (ntm_need <= 5) ntm = 5;
+  else if (ntm_need <= 9) ntm = 9;
+  else if (ntm_need <= 17) ntm = 17;
+  else return fail(CCS_ERR_UNSUPPORTED, "ccs_correlation_volume: disparity span %d exceeds 256", span);
+  a.sp = 16 * ntm;
+  const size_t esz = precision == CCS_BF16 ? 2 : 4;
+  a.pitch = precision == CCS_BF16 ? a.Cs : a.Cs + 1;
+  int NW = 4;
+  for (; NW >= 1; NW >>= 1) {
+    a.win = 16 * (NW - 1) + 16 * ntm;
+    const size_t bytes = (size_t)(16 * NW + a.win) * a.pitch * esz + (size_t)NW * 16 * a.sp * 4 + NW * 16 * 8;
+    if (bytes <= 160 * 1024) break;
+  }
+  CCS_CHECK(NW >= 1, CCS_ERR_UNSUPPORTED, "ccs_correlation_volume: tile does not fit in LDS");
+  hipStream_t st = h->stream;
+  if (precision == CCS_BF16) {
+    switch (ntm) {
+      case 3: return launch_volume<bf16, 3>(a, NW, st);
+      case 5: return launch_volume<bf16, 5>(a, NW, st);
+      case 9: return launch_volume<bf16, 9>(a, NW, st);
+      default: return launch_volume<bf16, 17>(a, NW, st);
+    }
+  }
+  switch (ntm) {
+    case 3: return launch_volume<float, 3>(a, NW, st);
+    case 5: return launch_volume<float, 5>(a, NW, st);
+    case 9: return launch_volume<float, 9>(a, NW, st);
+    default: return launch_volume<float, 17>(a, NW, st);
+  }
+}
+
+// Reads the reference's float disparity list to the host and checks that every
+// entry is an integer (the interpolating kernels of the reference are not offered).
+static int fetch_disparities(ccs_handle_t h, const float* d_disp, int n, std::vector<int>& out) {
+  std::vector<float> hd(n);
+  CCS_HIP(hipMemcpyAsync(hd.data(), d_disp, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+  CCS_HIP(hipStreamSynchronize(h->stream));
+  out.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const float r = std::round(hd[i]);
+    CCS_CHECK(std::fabs(hd[i] - r) <= 1e-3f, CCS_ERR_UNSUPPORTED,
+              "ccs_correlation: fractional disparity %g (interpolating correlation is not provided)", hd[i]);
+    out[i] = (int)r;
+  }
+  return CCS_OK;
+}
+
+extern "C" int ccs_correlation_forward(ccs_handle_t h, const float* input0, const float* input1,
+                                       const float* disparities, float* output, int in, int ic, int ih, int iw,
+                                       int len_disps, int memory_layout, int rect_corr) {
+  CCS_CHECK(h && input0 && input1 && disparities && output, CCS_ERR_ARG, "ccs_correlation_forward: NULL argument");
+  CCS_CHECK(in >= 1 && ic >= 1 && ih >= 1 && iw >= 1 && len_disps >= 1, CCS_ERR_ARG,
+            "ccs_correlation_forward: empty shape");
+  CCS_CHECK(memory_layout == CCS_LAYOUT_NHWC || memory_layout == CCS_LAYOUT_NCHW, CCS_ERR_ARG,
+            "ccs_correlation_forward: bad memory layout %d", memory_layout);
+  CCS_CHECK(rect_corr >= 0, CCS_ERR_ARG, "ccs_correlation_forward: negative rect_corr");
+  std::vector<int> disp;
+  int rc = fetch_disparities(h, disparities, len_disps, disp);
+  if (rc) return rc;
+  if ((rc = h->misc.reserve(sizeof(int) * len_disps))) return rc;
+  int* dd = (int*)h->misc.ptr;
+  CCS_HIP(hipMemcpyAsync(dd, disp.data(), sizeof(int) * len_disps, hipMemcpyHostToDevice, h->stream));
+  const long total = (long)in * ih * iw * len_disps;
+  k_corr_generic<<<ceil_div(total, 256), 256, 0, h->stream>>>(input0, input1, dd, output, in, ic, ih, iw, len_disps,
+                                                             memory_layout == CCS_LAYOUT_NHWC, rect_corr);
+  CCS_LAUNCH_CHECK();
+  CCS_HIP(hipStreamSynchronize(h->stream));
+  return CCS_OK;
+}
+
+extern "C" int ccs_correlation_backward(ccs_handle_t h, float* out_grad0, float* out_grad1, const float* in_grad,
+                                        const float* im0, const float* im1, const float* disparities, int n, int c,
+                                        int ih, int iw, int num_disps, int memory_layout) {
+  CCS_CHECK(h && out_grad0 && out_grad1 && in_grad && im0 && im1 && disparities, CCS_ERR_ARG,
+            "ccs_correlation_backward: NULL argument");
+  CCS_CHECK(n >= 1 && c >= 1 && ih >= 1 && iw >= 1 && num_disps >= 1, CCS_ERR_ARG,
+            "ccs_correlation_backward: empty shape");
+  CCS_CHECK(memory_layout == CCS_LAYOUT_NHWC || memory_layout == CCS_LAYOUT_NCHW, CCS_ERR_ARG,
+            "ccs_correlation_backward: bad memory layout");
+  std::vector<int> disp;
+  int rc = fetch_disparities(h, disparities, num_disps, disp);
+  if (rc) return rc;
+  if ((rc = h->misc.reserve(sizeof(int) * num_disps))) return rc;
+  int* dd = (int*)h->misc.ptr;
+  CCS_HIP(hipMemcpyAsync(dd, disp.data(), sizeof(int) * num_disps, hipMemcpyHostToDevice, h->stream));
+  const long total = (long)n * c * ih * iw;
+  k_corr_bwd_generic<<<ceil_div(total, 256), 256, 0, h->stream>>>(out_grad0, out_grad1, in_grad, im0, im1, dd, n, c,
+                                                                 ih, iw, num_disps,
+                                                                 memory_layout == CCS_LAYOUT_NHWC);
+  CCS_LAUNCH_CHECK();
+  CCS_HIP(hipStreamSynchronize(h->stream));
+  return CCS_OK;
+}
+
+extern "C" int ccs_correlation_volume_backward(ccs_handle_t h, const float* f0, const float* f1, int H, int W, int C,
+                                               int disp_min, int disp_step, int D, const float* G, float* df0,
+                                               float* df1) {
+  CCS_CHECK(h && f0 && f1 && G && df0 && df1, CCS_ERR_ARG, "ccs_correlation_volume_backward: NULL argument");
+  CCS_CHECK(H >= 1 && W >= 1 && C >= 1 && D >= 1, CCS_ERR_ARG, "ccs_correlation_volume_backward: empty shape");
+  const int Cs = feature_stride(C);
+  const long total = (long)H * W * Cs;
+  k_corr_volume_bwd<<<ceil_div(total, 256), 256, 0, h->stream>>>(f0, f1, H, W, C, Cs, -disp_min, -disp_step, D, G,
+                                                                df0, df1);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_softmax_xent(ccs_handle_t h, const float* S, int HW, int D, const int* target, double* loss,
+                                float* dS) {
+  CCS_CHECK(h && S && target && loss && dS && HW >= 1 && D >= 1, CCS_ERR_ARG, "ccs_softmax_xent: bad argument");
+  const int nb = 256;
+  int rc;
+  if ((rc = h->misc2.reserve(sizeof(double) * nb))) return rc;
+  double* part = (double*)h->misc2.ptr;
+  k_softmax_xent<<<nb, 256, 0, h->stream>>>(S, HW, D, target, dS, part);
+  k_sum_f64_corr<<<1, 64, 0, h->stream>>>(part, nb, loss);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}

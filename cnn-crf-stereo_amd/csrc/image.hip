@@ -1,0 +1,236 @@
+// Input preparation and decode kernels: mirror pad + normalisation, rgb->gray,
+// edge weights, argmin, label remap.
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+namespace ccs {
+
+__device__ __forceinline__ int reflect(int i, int n) {
+  if (n == 1) return 0;
+  const int period = 2 * n;
+  i %= period;
+  if (i < 0) i += period;
+  return i < n ? i : period - 1 - i;
+}
+
+// out[c][y][x] = img[c][reflect(y-p)][reflect(x-p)]  (stereo/stereo.cu:879-889).
+__global__ void k_pad(const float* __restrict__ img, int H, int W, int p, float* __restrict__ out) {
+  const int Wo = W + 2 * p, Ho = H + 2 * p;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  const int c = blockIdx.z;
+  if (x >= Wo) return;
+  out[((long)c * Ho + y) * Wo + x] =
+      img[((long)c * H + reflect(y - p, H)) * W + reflect(x - p, W)];
+}
+
+// Per-block partial sums of x and x^2 in float64; grid (nb, C).
+__global__ void k_moments_partial(const float* __restrict__ x, long n, double* __restrict__ part) {
+  const float* xc = x + (long)blockIdx.y * n;
+  double s = 0.0, q = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double v = xc[i];
+    s += v;
+    q += v * v;
+  }
+  __shared__ double ss[256], sq[256];
+  ss[threadIdx.x] = s;
+  sq[threadIdx.x] = q;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      ss[threadIdx.x] += ss[threadIdx.x + o];
+      sq[threadIdx.x] += sq[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * ((long)blockIdx.y * gridDim.x + blockIdx.x)] = ss[0];
+    part[2 * ((long)blockIdx.y * gridDim.x + blockIdx.x) + 1] = sq[0];
+  }
+}
+
+// mean = sum / N, std = sqrt(sum x^2 / N - mean^2) (population, stereo/utils.cu:160-237).
+__global__ void k_moments_final(const double* __restrict__ part, int nb, long n, double* __restrict__ stats,
+                                int C) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  double s = 0.0, q = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    s += part[2 * ((long)c * nb + b)];
+    q += part[2 * ((long)c * nb + b) + 1];
+  }
+  const double mean = s / (double)n;
+  const double var = q / (double)n - mean * mean;
+  stats[c] = mean;
+  stats[C + c] = var > 0.0 ? sqrt(var) : 0.0;
+}
+
+__global__ void k_normalize(float* __restrict__ x, long n, const double* __restrict__ stats, int C) {
+  const int c = blockIdx.y;
+  const double mean = stats[c];
+  const double sd = stats[C + c];
+  const float inv = sd > 0.0 ? (float)(1.0 / sd) : 1.0f;
+  const float m = (float)mean;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[(long)c * n + i] = (x[(long)c * n + i] - m) * inv;
+}
+
+__global__ void k_rgb2gray(const float* __restrict__ rgb, long n, float* __restrict__ gray) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  gray[i] = 0.2989f * rgb[i] + 0.5870f * rgb[n + i] + 0.1140f * rgb[2 * n + i];
+}
+
+constexpr int kMaxGaussTaps = 33;
+struct GaussTaps {
+  float k[kMaxGaussTaps];
+  int r;
+};
+
+__global__ void k_gauss_h(const float* __restrict__ in, int H, int W, GaussTaps t, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W) return;
+  float a = 0.f;
+  for (int i = -t.r; i <= t.r; ++i) a += t.k[i + t.r] * in[(long)y * W + reflect(x + i, W)];
+  out[(long)y * W + x] = a;
+}
+
+__device__ __forceinline__ float gauss_v_at(const float* __restrict__ tmp, int H, int W, const GaussTaps& t,
+                                            int x, int y) {
+  float a = 0.f;
+  for (int i = -t.r; i <= t.r; ++i) a += t.k[i + t.r] * tmp[(long)reflect(y + i, H) * W + x];
+  return a;
+}
+
+__device__ __forceinline__ float edge_fn(float d, float alpha, float beta, float lambda) {
+  return alpha > 0.f ? fmaxf(lambda * expf(-alpha * powf(d, beta)), 1e-6f) : lambda * d;
+}
+
+// Vertical blur + forward differences (stereo/stereo.cu:471-496, 525-550).
+__global__ void k_edge(const float* __restrict__ tmp, int H, int W, GaussTaps t, float alpha, float beta,
+                       float lambda, float* __restrict__ wx, float* __restrict__ wy) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W) return;
+  const float g = gauss_v_at(tmp, H, W, t, x, y);
+  const float gx = gauss_v_at(tmp, H, W, t, reflect(x + 1, W), y);
+  const float gy = gauss_v_at(tmp, H, W, t, x, reflect(y + 1, H));
+  wx[(long)y * W + x] = edge_fn(fabsf(gx - g), alpha, beta, lambda);
+  wy[(long)y * W + x] = edge_fn(fabsf(gy - g), alpha, beta, lambda);
+}
+
+// First strict minimum over D (ties -> lowest index), G lanes per pixel.
+template <int G>
+__global__ void k_argmin(const float* __restrict__ vol, long HW, int D, float dmin, float dstep,
+                         float* __restrict__ out) {
+  const int lane = threadIdx.x % G;
+  const long p = ((long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const bool valid = p < HW;
+  float best = INFINITY;
+  int bi = 0x7fffffff;
+  if (valid) {
+    const float* v = vol + p * D;
+    for (int k = lane; k < D; k += G) {
+      const float c = v[k];
+      if (c < best) {
+        best = c;
+        bi = k;
+      }
+    }
+  }
+  for (int o = G / 2; o > 0; o >>= 1) {
+    const float ob = __shfl_xor(best, o, G);
+    const int oi = __shfl_xor(bi, o, G);
+    if (ob < best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  if (valid && lane == 0) out[p] = dmin + (float)(bi == 0x7fffffff ? 0 : bi) * dstep;
+}
+
+__global__ void k_remap(const int* __restrict__ lab, long n, float dmin, float dstep, float* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)lab[i] * dstep + dmin;
+}
+
+}  // namespace ccs
+
+using namespace ccs;
+
+extern "C" int ccs_pad_normalize(ccs_handle_t h, const float* img, int C, int H, int W, int pad, float* out,
+                                 double* stats) {
+  CCS_CHECK(h && img && out, CCS_ERR_ARG, "ccs_pad_normalize: NULL argument");
+  CCS_CHECK(C >= 1 && C <= 64 && H >= 1 && W >= 1 && pad >= 0, CCS_ERR_ARG,
+            "ccs_pad_normalize: bad shape C=%d H=%d W=%d pad=%d", C, H, W, pad);
+  const int Ho = H + 2 * pad, Wo = W + 2 * pad;
+  const long n = (long)Ho * Wo;
+  const int nb = 128;
+  int rc = h->misc.reserve(sizeof(double) * (2 * (size_t)C * nb + 2 * C));
+  if (rc) return rc;
+  double* part = (double*)h->misc.ptr;
+  double* st = stats ? stats : part + 2 * (size_t)C * nb;
+  k_pad<<<dim3(ceil_div(Wo, 256), Ho, C), 256, 0, h->stream>>>(img, H, W, pad, out);
+  k_moments_partial<<<dim3(nb, C), 256, 0, h->stream>>>(out, n, part);
+  k_moments_final<<<1, 64, 0, h->stream>>>(part, nb, n, st, C);
+  k_normalize<<<dim3(256, C), 256, 0, h->stream>>>(out, n, st, C);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_rgb_to_gray(ccs_handle_t h, const float* rgb, int H, int W, float* gray) {
+  CCS_CHECK(h && rgb && gray && H > 0 && W > 0, CCS_ERR_ARG, "ccs_rgb_to_gray: bad argument");
+  const long n = (long)H * W;
+  k_rgb2gray<<<ceil_div(n, 256), 256, 0, h->stream>>>(rgb, n, gray);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_edge_weights(ccs_handle_t h, const float* gray, int H, int W, float sigma, float alpha,
+                                float beta, float lambda, float* wx, float* wy) {
+  CCS_CHECK(h && gray && wx && wy && H > 0 && W > 0, CCS_ERR_ARG, "ccs_edge_weights: bad argument");
+  CCS_CHECK(sigma > 0.f, CCS_ERR_ARG, "ccs_edge_weights: sigma must be positive");
+  GaussTaps t;
+  t.r = (int)std::ceil(3.0 * sigma);
+  CCS_CHECK(2 * t.r + 1 <= kMaxGaussTaps, CCS_ERR_UNSUPPORTED, "ccs_edge_weights: sigma %g too large", sigma);
+  double s = 0.0;
+  std::vector<double> k(2 * t.r + 1);
+  for (int i = -t.r; i <= t.r; ++i) {
+    k[i + t.r] = std::exp(-(double)(i * i) / (2.0 * (double)sigma * (double)sigma));
+    s += k[i + t.r];
+  }
+  for (int i = 0; i < 2 * t.r + 1; ++i) t.k[i] = (float)(k[i] / s);
+  int rc = h->misc2.reserve(sizeof(float) * (size_t)H * W);
+  if (rc) return rc;
+  float* tmp = (float*)h->misc2.ptr;
+  k_gauss_h<<<dim3(ceil_div(W, 256), H), 256, 0, h->stream>>>(gray, H, W, t, tmp);
+  k_edge<<<dim3(ceil_div(W, 256), H), 256, 0, h->stream>>>(tmp, H, W, t, alpha, beta, lambda, wx, wy);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_argmin(ccs_handle_t h, const float* cost, int H, int W, int D, float disp_min,
+                          float disp_step, float* out) {
+  CCS_CHECK(h && cost && out && H > 0 && W > 0 && D > 0, CCS_ERR_ARG, "ccs_argmin: bad argument");
+  const long HW = (long)H * W;
+  if (D >= 64) {
+    k_argmin<64><<<ceil_div(HW * 64, 256), 256, 0, h->stream>>>(cost, HW, D, disp_min, disp_step, out);
+  } else {
+    k_argmin<8><<<ceil_div(HW * 8, 256), 256, 0, h->stream>>>(cost, HW, D, disp_min, disp_step, out);
+  }
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_remap(ccs_handle_t h, const int* labels, int H, int W, float disp_min, float disp_step,
+                         float* out) {
+  CCS_CHECK(h && labels && out && H > 0 && W > 0, CCS_ERR_ARG, "ccs_remap: bad argument");
+  const long n = (long)H * W;
+  k_remap<<<ceil_div(n, 256), 256, 0, h->stream>>>(labels, n, disp_min, disp_step, out);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}

@@ -1,0 +1,203 @@
+/*
+ * ccs.h -- C ABI of the MI355X-native CNN-CRF stereo hot path.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every device pointer is
+ * caller-owned HBM; work is enqueued on the handle's HIP stream and nothing
+ * blocks unless a function says so.  Functions return CCS_OK (0) or a
+ * negative status; ccs_last_error() gives the message of the calling thread's
+ * last failure.  Each entry point names the reference interface it replaces
+ * (paths relative to the VLOGroup/cnn-crf-stereo source tree).
+ *
+ * Layouts
+ *   image        C x H x W float32                      (reference NCHW, N = 1)
+ *   feature map  H x W x Cs, Cs = ccs_feature_stride(C) (channel-last, channels
+ *                C..Cs-1 hold zeros), element type float32 or bfloat16
+ *   cost volume  H x W x D float32, disparity fastest   (reference NHWC)
+ *   edge weight  H x W float32; wx[y][x] couples (x,y)-(x+1,y), wy[y][x] couples (x,y)-(x,y+1)
+ *   labels       H x W int32
+ */
+#ifndef CCS_H
+#define CCS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ccs_handle_s* ccs_handle_t;
+
+enum {
+  CCS_OK = 0,
+  CCS_ERR_ARG = -1,         /* invalid argument or size                       */
+  CCS_ERR_UNSUPPORTED = -2, /* valid in the reference but not offered here     */
+  CCS_ERR_HIP = -3,         /* HIP runtime error                                */
+  CCS_ERR_ALLOC = -4,       /* device allocation failed                         */
+  CCS_ERR_IO = -5,          /* file missing or malformed                        */
+  CCS_ERR_SHAPE = -6        /* tensor shape does not match the operator         */
+};
+
+enum { CCS_LAYOUT_NCHW = 0, CCS_LAYOUT_NHWC = 1 };    /* iu::TensorGpu_32f::MemoryLayout */
+enum { CCS_FP32 = 0, CCS_BF16 = 1 };                  /* arithmetic of conv / correlation */
+enum { CCS_EPILOGUE_RAW = 0, CCS_EPILOGUE_NEG_SOFTMAX = 1 };
+
+const char* ccs_last_error(void);
+const char* ccs_version(void);
+
+/* Handle: a HIP stream plus grow-only scratch memory owned by the handle.
+ * Replaces cudnnCreate / cudnnDestroy (stereo/stereonet.cpp:38,43).
+ * stream == NULL uses the null stream. */
+int ccs_create(ccs_handle_t* handle, void* hip_stream);
+int ccs_destroy(ccs_handle_t handle);
+int ccs_set_stream(ccs_handle_t handle, void* hip_stream);
+int ccs_synchronize(ccs_handle_t handle);
+
+/* Channel stride of a feature map with C channels (C rounded up to a multiple of 8). */
+int ccs_feature_stride(int C);
+
+/* ---- input preparation ------------------------------------------------- */
+
+/* Mirror padding + per-channel zero mean / unit population std of one image.
+ * Replaces padColorImage / padding_kernel (stereo/stereo.cu:865-889,1022-1044)
+ * followed by makeRgbZeroMean + makeRgbUnitStd (stereo/utils.cu:169-237) or
+ * makeZeroMean + makeUnitStd (stereo/utils.cu:242-287).
+ * img: C x H x W float32; out: C x (H+2p) x (W+2p) float32.
+ * stats (optional, device, 2*C doubles) receives mean[0..C) and std[0..C). */
+int ccs_pad_normalize(ccs_handle_t h, const float* img, int C, int H, int W, int pad,
+                      float* out, double* stats);
+
+/* 0.2989 R + 0.5870 G + 0.1140 B of a 3 x H x W image: stereo/stereo.cu:853-863. */
+int ccs_rgb_to_gray(ccs_handle_t h, const float* rgb, int H, int W, float* gray);
+
+/* ---- feature CNN --------------------------------------------------------- */
+
+/* Fused Convolution + Bias + Activation(tanh) of one tower layer.  Replaces
+ * Convolution::forward (stereo/operators/convolution.cpp:146-152, CUDNN_CONVOLUTION,
+ * i.e. the filter is flipped, pad 0, stride 1), Bias::forward
+ * (stereo/operators/bias.cpp:58-66) and Activation::forward
+ * (stereo/operators/activation.cpp:33-40).
+ * x: H x W x ccs_feature_stride(C) (or, with x_layout == CCS_LAYOUT_NCHW and
+ *    C <= 8, a C x H x W float32 image); y: (H-KH+1) x (W-KW+1) x ccs_feature_stride(K).
+ * w: K x C x KH x KW float32 (the reference's npz layout), b: K float32 (or NULL).
+ * precision CCS_BF16: x, y are bfloat16 and the MFMA accumulates in float32.
+ * act: 1 = tanh, 0 = identity.  Weights are repacked into handle scratch on
+ * every call; ccs_conv_pack_weights + ccs_conv_forward_packed avoid that. */
+int ccs_conv2d_forward(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C,
+                       const float* w, const float* b, int K, int KH, int KW, int act,
+                       int precision, void* y);
+
+size_t ccs_conv_packed_bytes(int K, int C, int KH, int KW, int precision);
+int ccs_conv_pack_weights(ccs_handle_t h, const float* w, int K, int C, int KH, int KW,
+                          int precision, void* packed);
+int ccs_conv_forward_packed(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C,
+                            const void* packed, const float* b, int K, int KH, int KW, int act,
+                            int precision, void* y);
+
+/* Adjoint of ccs_conv2d_forward (float32 feature maps, tanh recomputed from x).
+ * dy: (H-KH+1) x (W-KW+1) x stride(K) gradient w.r.t. the activated output.
+ * dx: H x W x stride(C) (may be NULL), dw: K x C x KH x KW, db: K (may be NULL).
+ * No reference counterpart (the reference has no convolution backward). */
+int ccs_conv2d_backward(ccs_handle_t h, const float* x, int H, int W, int C, const float* w,
+                        const float* b, int K, int KH, int KW, int act, const float* dy,
+                        float* dx, float* dw, float* db);
+
+/* ---- correlation cost volume -------------------------------------------- */
+
+/* StereoCorrelation::forward static (stereo/operators/stereocorrelation.h:38-40,
+ * stereocorrelation.cu:386-408): same argument order and meaning.  input0/1 are
+ * N x C x H x W float32, disparities is a device array of the reference's
+ * integer-valued offsets (-d for left->right), output is N x H x W x D
+ * (memory_layout CCS_LAYOUT_NHWC) or N x D x H x W (CCS_LAYOUT_NCHW).
+ * Offsets leaving the image give 0.  Like the reference it reads the
+ * disparities back to the host, so it synchronises the stream. */
+int ccs_correlation_forward(ccs_handle_t h, const float* input0, const float* input1,
+                            const float* disparities, float* output, int in, int ic, int ih,
+                            int iw, int len_disps, int memory_layout, int rect_corr);
+
+/* StereoCorrelation::backward static (stereo/operators/stereocorrelation.h:42-44,
+ * stereocorrelation.cu:437-474): dIn0/1 (N x C x H x W) from the output gradient
+ * in_grad (N x H x W x D for CCS_LAYOUT_NHWC).  Gathers by disparity band, no
+ * atomics; synchronises like the reference. */
+int ccs_correlation_backward(ccs_handle_t h, float* out_grad0, float* out_grad1,
+                             const float* in_grad, const float* im0, const float* im1,
+                             const float* disparities, int n, int c, int ih, int iw,
+                             int num_disps, int memory_layout);
+
+/* Fast path of StereoCorrelation + Softmax + negateTensor
+ * (stereo/stereonet.cpp:301-318): banded MFMA correlation of two feature maps
+ * (H x W x stride(C), float32 or bfloat16) over the offsets
+ * -(disp_min + k disp_step), k < D, with the -softmax over D fused
+ * (epilogue CCS_EPILOGUE_NEG_SOFTMAX) or the raw correlation (CCS_EPILOGUE_RAW).
+ * out: H x W x D float32.  Stream-ordered, no host synchronisation. */
+int ccs_correlation_volume(ccs_handle_t h, const void* f0, const void* f1, int precision,
+                           int H, int W, int C, int disp_min, int disp_step, int D,
+                           int epilogue, float* out);
+
+/* Adjoint of the fast path on float32 feature maps: G = dLoss/dS (H x W x D,
+ * gradient of the raw correlation) -> df0, df1 (H x W x stride(C)). */
+int ccs_correlation_volume_backward(ccs_handle_t h, const float* f0, const float* f1, int H,
+                                    int W, int C, int disp_min, int disp_step, int D,
+                                    const float* G, float* df0, float* df1);
+
+/* Cross entropy of softmax_D(S) at target labels (target < 0 ignored):
+ * loss (device, 1 double) and dS = softmax - onehot (H x W x D). */
+int ccs_softmax_xent(ccs_handle_t h, const float* S, int HW, int D, const int* target,
+                     double* loss, float* dS);
+
+/* ---- CRF ----------------------------------------------------------------- */
+
+/* Edge weights: Gaussian(sigma) of the gray image, then forward differences with
+ * a mirrored border, w = alpha > 0 ? max(lambda exp(-alpha |d|^beta), 1e-6) : lambda |d|.
+ * Replaces calcEdgeImage (stereo/stereo.cu:1090-1095, 471-496, 525-550). */
+int ccs_edge_weights(ccs_handle_t h, const float* gray, int H, int W, float sigma, float alpha,
+                     float beta, float lambda, float* wx, float* wy);
+
+/* SlackProp::forward static, old interface (stereo/operators/slackprop.h:43-44,
+ * slackprop.cpp:111-139): same argument order; delta = 2 as in the reference.
+ * cost_volume: ih x iw x ic (ic = number of labels), output: ih x iw labels
+ * written as float.  *lower_bound receives the dual bound.  Synchronises. */
+int ccs_slackprop_forward(ccs_handle_t h, const float* cost_volume, const float* wx,
+                          const float* wy, float* output, int in, int ic, int ih, int iw,
+                          float P1, float P2, int max_iter, float* lower_bound);
+
+/* SlackProp CRF inference (the restated dual minorize-maximize over row and column
+ * chains, float32 min-sum, float64 bound bookkeeping).  Call sites it replaces:
+ * Stereo::computeSlackPropagation (stereo/stereo.cpp:129-162) and
+ * SlackProp::forwardVolumeOut (stereo/operators/slackprop.cpp:142-173).
+ * cost: H x W x D; labels: H x W int32.  unaries_out (optional, H x W x D) receives
+ * the final row unaries (the reparametrised volume).  sums (optional, device,
+ * 4*iters+1 doubles) receives the per-half-step chain minima S_0..S_{2 iters}
+ * followed by the kept constants K_0..K_{2 iters - 1}; the bound after half-step
+ * j >= 1 is K_{j-1} + S_j.  lower_bound (optional, device double) receives the
+ * final bound.  Stream-ordered, no host synchronisation. */
+int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H,
+                  int W, int D, float L1, float L2, int delta, int iters, int* labels,
+                  float* unaries_out, double* sums, double* lower_bound);
+
+/* CRF energy of a labelling, accumulated in float64 (device double). */
+int ccs_crf_energy(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H,
+                   int W, int D, const int* labels, float L1, float L2, int delta,
+                   double* energy);
+
+/* Structured-SVM subgradient of E(x*) - E(xhat) w.r.t. the unaries (dC, H x W x D,
+ * overwritten), L1/L2 (dL, device, 2 doubles) and the edge weights (dwx, dwy,
+ * H x W, optional).  No reference counterpart (training lived outside the repo). */
+int ccs_crf_ssvm_grad(ccs_handle_t h, const int* xhat, const int* xstar, const float* wx,
+                      const float* wy, int H, int W, int D, float L1, float L2, int delta,
+                      float* dC, double* dL, float* dwx, float* dwy);
+
+/* ---- decode -------------------------------------------------------------- */
+
+/* First strict minimum over D per pixel -> disp_min + index * disp_step
+ * (stereo/stereo.cu:348-370, 975-994).  out: H x W float32. */
+int ccs_argmin(ccs_handle_t h, const float* cost, int H, int W, int D, float disp_min,
+               float disp_step, float* out);
+
+/* label * disp_step + disp_min (stereo/stereo.cu:841-850, 1229-1246). */
+int ccs_remap(ccs_handle_t h, const int* labels, int H, int W, float disp_min, float disp_step,
+              float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
