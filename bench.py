@@ -1,0 +1,315 @@
+#!/usr/bin/env python3
+"""Benchmark of the CNN-CRF stereo hot path on one MI355X.
+
+A step is one stereo frame through the whole path with the inputs already in
+HBM: pad + normalise, the two 7-layer feature towers, correlation + -softmax,
+edge weights, SlackProp CRF and the label -> disparity map.  The workload is
+the 1242 x 375 grayscale random-dot pair with D = 128, a 7-layer 100-channel
+CNN of 3x3 layers and 8 CRF iterations; the BF16-MFMA form is the measured
+value and the FP32 form runs beside it on the same GPU.
+
+Prints one JSON line (see --help).  Per-launch HIP events bracket every kernel
+launch inside the timed steps on the handle's stream; the roofline entry is
+the dominant kernel's algorithmic work per launch over its mean duration.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "cnn-crf-stereo_amd")
+
+METRIC = "ms/frame + pixel·disp/s at 1242x375 D=128, BF16-MFMA vs FP32, same GPU"
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp32": 157.3}
+
+
+def _load_package():
+    if "cnn_crf_stereo_amd" in sys.modules:
+        return sys.modules["cnn_crf_stereo_amd"]
+    spec = importlib.util.spec_from_file_location("cnn_crf_stereo_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["cnn_crf_stereo_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class Launches:
+    """HIP events bracketing kernel launches, grouped by kernel class."""
+
+    def __init__(self, enabled):
+        import torch
+
+        self.torch = torch
+        self.enabled = enabled
+        self.records = []  # (cls, work dict, ev0, ev1)
+
+    def run(self, cls, work, fn):
+        if not self.enabled:
+            return fn()
+        e0 = self.torch.cuda.Event(enable_timing=True)
+        e1 = self.torch.cuda.Event(enable_timing=True)
+        e0.record()
+        r = fn()
+        e1.record()
+        self.records.append((cls, work, e0, e1))
+        return r
+
+    def summary(self):
+        self.torch.cuda.synchronize()
+        out = {}
+        for cls, work, e0, e1 in self.records:
+            d = out.setdefault(cls, dict(ms=0.0, launches=0, flops=0.0, bytes=0.0))
+            d["ms"] += e0.elapsed_time(e1)
+            d["launches"] += 1
+            d["flops"] += work.get("flops", 0.0)
+            d["bytes"] += work.get("bytes", 0.0)
+        return out
+
+
+def run_frame(ops, h, frame, left, right, L, A):
+    """One frame, every kernel launch bracketed by events (when enabled)."""
+    H, W = left.shape[1:]
+    D = frame.D
+    out = {}
+    pad = frame.pad
+    esize = 2 if frame.tower.precision == "bf16" else 4
+    xs = []
+    for img in (left, right):
+        xs.append(L.run("normalize", {"bytes": 4.0 * img.numel() * 3}, lambda img=img: ops.pad_normalize(h, img, pad)))
+    feats = []
+    for x in xs:
+        y = x
+        for li, layer in enumerate(frame.tower.layers):
+            if li == 0:
+                Hi, Wi = y.shape[1:]
+            else:
+                Hi, Wi = y.shape[:2]
+            Ho, Wo = Hi - layer.KH + 1, Wi - layer.KW + 1
+            flops = 2.0 * layer.K * layer.C * layer.KH * layer.KW * Ho * Wo
+            cls = "conv_first" if li == 0 else "conv"
+            lay = "nchw" if li == 0 else "nhwc"
+            y = L.run(cls, {"flops": flops}, lambda y=y, layer=layer, lay=lay: layer(h, y, layout=lay))
+        feats.append(y)
+    C = frame.C
+    cs = feats[0].shape[2]
+    corr_bytes = 2.0 * H * W * cs * esize + 4.0 * H * W * D
+    cost = L.run("correlation", {"bytes": corr_bytes, "flops": 2.0 * C * H * W * D},
+                 lambda: ops.correlation_volume(h, feats[0], feats[1], C, frame.disp_min, frame.disp_step, D))
+    wx, wy = L.run("edge_weights", {"bytes": 12.0 * H * W},
+                   lambda: ops.edge_weights(h, left[0], frame.sigma, frame.alpha, frame.beta, frame.lam))
+    V = 4.0 * H * W * D
+    A.copy_(cost)
+    import torch
+
+    labels = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    for _ in range(frame.iters):
+        L.run("crf_rows", {"bytes": 2 * V}, lambda: ops.slackprop_sweep(h, cost, A, wx, wy, frame.L1, frame.L2,
+                                                                        frame.delta, 0))
+        L.run("crf_cols", {"bytes": 3 * V}, lambda: ops.slackprop_sweep(h, cost, A, wx, wy, frame.L1, frame.L2,
+                                                                        frame.delta, 1))
+    L.run("crf_decode", {"bytes": V + 4.0 * H * W},
+          lambda: ops.slackprop_sweep(h, cost, A, wx, wy, frame.L1, frame.L2, frame.delta, 2, labels=labels))
+    disp = L.run("remap", {"bytes": 8.0 * H * W}, lambda: ops.remap(h, labels, frame.disp_min, frame.disp_step))
+    out.update(cost=cost, labels=labels, disparity=disp)
+    return out
+
+
+def time_form(pkg, h, data, spec, params, precision, steps, warmup):
+    import torch
+    from cnn_crf_stereo_amd import ops, pipeline
+
+    frame = pipeline.Frame(h, params, precision, spec["D"], iters=spec["iters"])
+    left = torch.as_tensor(data["left"], device="cuda")
+    right = torch.as_tensor(data["right"], device="cuda")
+    H, W = left.shape[1:]
+    A = torch.empty((H, W, spec["D"]), dtype=torch.float32, device="cuda")
+    for _ in range(warmup):
+        run_frame(ops, h, frame, left, right, Launches(False), A)
+    torch.cuda.synchronize()
+    L = Launches(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = run_frame(ops, h, frame, left, right, L, A)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    return ms, L.summary(), out
+
+
+def bad_pixel_rate(labels, gt, D, border=0):
+    import numpy as np
+
+    lab = labels.astype(np.float64)
+    err = np.abs(lab - gt) > 1.0
+    if border:
+        err = err[:, border:]
+    return float(err.mean())
+
+
+def cpu_baseline(data, spec, params, pad, rows):
+    """The FP64 oracle of the same pipeline on a band of `rows` image rows."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    H = data["left"].shape[1]
+    rows = H if rows <= 0 or rows > H else rows
+    y0 = max(0, H // 2 - rows // 2)
+    band = slice(y0, y0 + rows)
+    left = data["left"][:, band, :].astype(np.float64)
+    right = data["right"][:, band, :].astype(np.float64)
+    D = spec["D"]
+    t0 = time.perf_counter()
+    feats = []
+    for img in (left, right):
+        x, _, _ = O.normalize(O.pad_symmetric(img, pad))
+        for (w, b) in params:
+            x = O.conv2d(x, w, b, True)
+        feats.append(x)
+    S = O.correlation(feats[0], feats[1], -np.arange(D))
+    cost = O.neg_softmax(S)
+    wx, wy = O.edge_weights(left[0], 0.7, 10.0, 1.0, 1.0)
+    r = O.slackprop(cost, wx, wy, 0.5, 2.0, 2, spec["iters"])
+    dt = time.perf_counter() - t0
+    Wd = left.shape[2]
+    return dict(seconds=dt, rows=rows, y0=y0, pd=rows * Wd * D, labels=r["labels"], lb=r["lb"],
+                left=left.astype(np.float32), right=right.astype(np.float32))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="rds-1242x375-d128")
+    ap.add_argument("--arch", default="all3x3", choices=["all3x3", "reference"])
+    ap.add_argument("--layers", type=int, default=7)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="rows of the CPU-baseline band (0 = the whole frame)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d, WORLD_SIZE %d)"
+              % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    pkg = _load_package()
+    from cnn_crf_stereo_amd import capi, synth
+
+    torch.cuda.set_device(0)
+    spec = dict(synth.WORKLOADS[args.workload])
+    data = synth.stereo_pair(args.workload)
+    layers, pad = synth.architecture(args.layers, spec["C"], args.arch)
+    params = synth.glorot_weights(layers)
+    h = capi.Handle()
+    H, W, D = spec["H"], spec["W"], spec["D"]
+    pd_per_frame = float(H * W * D)
+
+    ms_bf16, launches_bf16, out_bf16 = time_form(pkg, h, data, spec, params, "bf16", args.steps, args.warmup)
+    result_extra = {}
+    if not args.no_fp32:
+        ms_fp32, launches_fp32, out_fp32 = time_form(pkg, h, data, spec, params, "fp32", args.steps, args.warmup)
+        result_extra["fp32_ms_per_frame"] = round(ms_fp32, 4)
+        result_extra["fp32_pixel_disp_per_s"] = pd_per_frame / (ms_fp32 / 1e3)
+        result_extra["speedup_bf16_vs_fp32"] = round(ms_fp32 / ms_bf16, 3)
+        result_extra["stages_ms_fp32"] = {k: round(v["ms"] / args.steps, 4) for k, v in launches_fp32.items()}
+
+    gt = data["disp"]
+    bad = {"bf16": bad_pixel_rate(out_bf16["labels"].cpu().numpy(), gt, D)}
+    if not args.no_fp32:
+        bad["fp32"] = bad_pixel_rate(out_fp32["labels"].cpu().numpy(), gt, D)
+        lab_b = out_bf16["labels"].cpu().numpy()
+        lab_f = out_fp32["labels"].cpu().numpy()
+        result_extra["labels_differ_bf16_vs_fp32"] = float((lab_b != lab_f).mean())
+    result_extra["bad_px_gt_gt1"] = bad
+
+    # Dominant kernel class of the BF16 form and its roofline.
+    stages = {k: v["ms"] / args.steps for k, v in launches_bf16.items()}
+    dom = max(launches_bf16, key=lambda k: launches_bf16[k]["ms"])
+    d = launches_bf16[dom]
+    mean_ms = d["ms"] / d["launches"]
+    if d["flops"] > 0 and dom.startswith("conv"):
+        per_launch = d["flops"] / d["launches"]
+        achieved = per_launch / (mean_ms / 1e3) / 1e12
+        roof = dict(kernel=dom, bound="mfma", achieved=round(achieved, 2), peak=MFMA_PEAK_TFLOPS["bf16"],
+                    unit="TFLOP/s", frac=round(achieved / MFMA_PEAK_TFLOPS["bf16"], 4), traffic=None,
+                    work_per_launch=per_launch, mean_launch_ms=round(mean_ms, 5))
+    else:
+        per_launch = d["bytes"] / d["launches"]
+        achieved = per_launch / (mean_ms / 1e3) / 1e9
+        roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, work_per_launch=per_launch,
+                    mean_launch_ms=round(mean_ms, 5))
+    kernels = {}
+    for k, v in launches_bf16.items():
+        m = v["ms"] / v["launches"]
+        e = dict(launches_per_frame=v["launches"] // args.steps, mean_launch_ms=round(m, 5))
+        if v["flops"] and k.startswith("conv"):
+            e["tflops"] = round(v["flops"] / v["launches"] / (m / 1e3) / 1e12, 2)
+        if v["bytes"]:
+            e["gbs"] = round(v["bytes"] / v["launches"] / (m / 1e3) / 1e9, 1)
+        kernels[k] = e
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(data, spec, params, pad, args.cpu_rows)
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = dict(value=cb["pd"] / cb["seconds"], unit="pixel*disp/s", cores=cores, kind="port",
+                   sample="FP64 oracle of every stage on a %dx%d band (rows %d-%d) of the same pair, D=%d, "
+                          "%d CRF iterations: %.2f s" % (W, cb["rows"], cb["y0"], cb["y0"] + cb["rows"] - 1, D,
+                                                          spec["iters"], cb["seconds"]))
+        # The same band through both GPU forms, compared with the oracle's labels.
+        from cnn_crf_stereo_amd import pipeline
+
+        band_l = torch.as_tensor(cb["left"], device="cuda")
+        band_r = torch.as_tensor(cb["right"], device="cuda")
+        gt_band = gt[cb["y0"]:cb["y0"] + cb["rows"]]
+        band = {"fp64_oracle": bad_pixel_rate(cb["labels"], gt_band, D)}
+        for prec in (["bf16"] + ([] if args.no_fp32 else ["fp32"])):
+            fr = pipeline.Frame(h, params, prec, D, iters=spec["iters"])
+            o = fr(band_l, band_r)
+            lab = o["labels"].cpu().numpy()
+            band[prec] = bad_pixel_rate(lab, gt_band, D)
+            band[prec + "_labels_equal_oracle"] = float((lab == cb["labels"]).mean())
+        result_extra["bad_px_band_vs_oracle"] = band
+
+    line = {
+        "metric": METRIC,
+        "value": pd_per_frame / (ms_bf16 / 1e3),
+        "unit": "pixel*disp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_bf16, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": "%s: grayscale random-dot pair %dx%d, D=%d, 7-layer 100-channel CNN (%s), "
+                               "%d SlackProp iterations" % (args.workload, W, H, D, args.arch, spec["iters"]),
+                   "form": "BF16-MFMA CNN + correlation (FP32 accumulate), FP32 CRF min-sum, FP64 bound"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "kernels": kernels,
+    }
+    line.update(result_extra)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

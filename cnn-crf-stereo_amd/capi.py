@@ -43,6 +43,7 @@ SIGNATURES = {
     "ccs_slackprop_forward": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _F, _I, ctypes.POINTER(_F)]),
     "ccs_slackprop": (_I, [_P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _I, _P, _P, _P, _P]),
     "ccs_slackprop_decode_gap": (_I, [_P, _P, _P, _I, _I, _I, _F, _F, _I, _P, _P]),
+    "ccs_slackprop_sweep": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _I, _P, _P, _P]),
     "ccs_crf_energy": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _F, _F, _I, _P]),
     "ccs_crf_ssvm_grad": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _P, _P, _P, _P]),
     "ccs_argmin": (_I, [_P, _P, _I, _I, _I, _F, _F, _P]),

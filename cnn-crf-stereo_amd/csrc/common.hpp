@@ -55,7 +55,8 @@ struct ccs_handle_s {
   ccs::Scratch conv_w;     // packed conv weights (non-packed entry point)
   ccs::Scratch crf_a;      // CRF row unaries
   ccs::Scratch crf_msg;    // CRF backward messages
-  ccs::Scratch crf_small;  // per-chain doubles, reductions
+  ccs::Scratch crf_small;  // per-chain doubles
+  ccs::Scratch crf_sums;   // per-half-step sums of ccs_slackprop
   ccs::Scratch misc;       // small temporaries
   ccs::Scratch misc2;
 };
@@ -84,35 +85,45 @@ __device__ __forceinline__ float dpp_f(float old, float src) {
                                                     ROW_MASK, BANK_MASK, BOUND_CTRL));
 }
 
-// Minimum over the 64 lanes, broadcast to every lane.
+template <int CTRL>
+__device__ __forceinline__ float dpp_rot(float x) {  // full masks: every lane has a source
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+
+// Minimum over the 64 lanes, left in every lane: rotations inside each row of
+// 16 lanes (quad_perm, row_ror:4, row_ror:8), then v_permlane16_swap and
+// v_permlane32_swap (gfx950) combine the rows -- no readlane, no partial masks.
 __device__ __forceinline__ float wave_min(float x) {
-  x = fminf(x, dpp_f<0xb1>(x, x));
-  x = fminf(x, dpp_f<0x4e>(x, x));
-  x = fminf(x, dpp_f<0x114>(x, x));  // row_shr:4
-  x = fminf(x, dpp_f<0x118>(x, x));  // row_shr:8
-  x = fminf(x, dpp_f<0x142, 0xa>(x, x));  // row_bcast:15
-  x = fminf(x, dpp_f<0x143, 0xc>(x, x));  // row_bcast:31
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+  x = fminf(x, dpp_rot<0xb1>(x));
+  x = fminf(x, dpp_rot<0x4e>(x));
+  x = fminf(x, dpp_rot<0x124>(x));
+  x = fminf(x, dpp_rot<0x128>(x));
+  const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  x = fminf(__uint_as_float(p[0]), __uint_as_float(p[1]));
+  const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return fminf(__uint_as_float(q[0]), __uint_as_float(q[1]));
 }
 
 __device__ __forceinline__ float wave_max(float x) {
-  x = fmaxf(x, dpp_f<0xb1>(x, x));
-  x = fmaxf(x, dpp_f<0x4e>(x, x));
-  x = fmaxf(x, dpp_f<0x114>(x, x));
-  x = fmaxf(x, dpp_f<0x118>(x, x));
-  x = fmaxf(x, dpp_f<0x142, 0xa>(x, x));
-  x = fmaxf(x, dpp_f<0x143, 0xc>(x, x));
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+  x = fmaxf(x, dpp_rot<0xb1>(x));
+  x = fmaxf(x, dpp_rot<0x4e>(x));
+  x = fmaxf(x, dpp_rot<0x124>(x));
+  x = fmaxf(x, dpp_rot<0x128>(x));
+  const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  x = fmaxf(__uint_as_float(p[0]), __uint_as_float(p[1]));
+  const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return fmaxf(__uint_as_float(q[0]), __uint_as_float(q[1]));
 }
 
 __device__ __forceinline__ float wave_sum(float x) {
-  x += dpp_f<0xb1, 0xf, 0xf, true>(0.f, x);
-  x += dpp_f<0x4e, 0xf, 0xf, true>(0.f, x);
-  x += dpp_f<0x114, 0xf, 0xf, true>(0.f, x);
-  x += dpp_f<0x118, 0xf, 0xf, true>(0.f, x);
-  x += dpp_f<0x142, 0xa, 0xf, false>(0.f, x);
-  x += dpp_f<0x143, 0xc, 0xf, false>(0.f, x);
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+  x += dpp_rot<0xb1>(x);
+  x += dpp_rot<0x4e>(x);
+  x += dpp_rot<0x124>(x);
+  x += dpp_rot<0x128>(x);
+  const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  x = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(q[0]) + __uint_as_float(q[1]);
 }
 
 // Value of the previous lane (lane 0 gets `fill`) / next lane (lane 63 gets `fill`).

@@ -119,6 +119,13 @@ __device__ __forceinline__ float local_min(const Vec<LPL>& x) {
 // near(s) = min over 0 < |s - t| < delta of x(t)  (+inf if delta == 1).
 template <int LPL>
 __device__ __forceinline__ Vec<LPL> near_min(const Vec<LPL>& x, int delta) {
+  if (delta == 2) {  // the common case, straight-line
+    const Vec<LPL> l = shift_label(x, -1, kInfF), r = shift_label(x, +1, kInfF);
+    Vec<LPL> o;
+#pragma unroll
+    for (int j = 0; j < LPL; ++j) o.v[j] = fminf(l.v[j], r.v[j]);
+    return o;
+  }
   Vec<LPL> acc;
 #pragma unroll
   for (int j = 0; j < LPL; ++j) acc.v[j] = kInfF;
@@ -134,6 +141,13 @@ __device__ __forceinline__ Vec<LPL> near_min(const Vec<LPL>& x, int delta) {
 
 template <int LPL>
 __device__ __forceinline__ Vec<LPL> near_max(const Vec<LPL>& x, int delta) {
+  if (delta == 2) {
+    const Vec<LPL> l = shift_label(x, -1, -kInfF), r = shift_label(x, +1, -kInfF);
+    Vec<LPL> o;
+#pragma unroll
+    for (int j = 0; j < LPL; ++j) o.v[j] = fmaxf(l.v[j], r.v[j]);
+    return o;
+  }
   Vec<LPL> acc;
 #pragma unroll
   for (int j = 0; j < LPL; ++j) acc.v[j] = -kInfF;
@@ -160,96 +174,176 @@ __device__ __forceinline__ Vec<LPL> minconv(const Vec<LPL>& U, float umin, float
   return m;
 }
 
-template <int LPL, int MODE>
+// Node loads never sit under divergent control flow (that would make the
+// compiler drain every outstanding load at the join): with FULL (D == 64*LPL)
+// every lane owns LPL labels and loads them as one vector; otherwise the lane
+// loads clamped addresses element by element and masks the labels >= D.
+template <int LPL, bool FULL>
+__device__ __forceinline__ Vec<LPL> ld_node(const float* __restrict__ base, int lane, int D, float fill) {
+  Vec<LPL> out;
+  if constexpr (FULL) {
+    if constexpr (LPL == 4) {
+      const float4 t = *reinterpret_cast<const float4*>(base + lane * 4);
+      out.v[0] = t.x; out.v[1] = t.y; out.v[2] = t.z; out.v[3] = t.w;
+    } else if constexpr (LPL == 2) {
+      const float2 t = *reinterpret_cast<const float2*>(base + lane * 2);
+      out.v[0] = t.x; out.v[1] = t.y;
+    } else {
+      out.v[0] = base[lane];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < LPL; ++j) {
+      const int s = lane * LPL + j;
+      const float v = base[s < D ? s : D - 1];
+      out.v[j] = s < D ? v : fill;
+    }
+  }
+  return out;
+}
+
+template <int LPL, bool FULL>
+__device__ __forceinline__ void st_node(float* __restrict__ base, int lane, int D, const Vec<LPL>& x) {
+  if constexpr (FULL) {
+    if constexpr (LPL == 4) {
+      *reinterpret_cast<float4*>(base + lane * 4) = make_float4(x.v[0], x.v[1], x.v[2], x.v[3]);
+    } else if constexpr (LPL == 2) {
+      *reinterpret_cast<float2*>(base + lane * 2) = make_float2(x.v[0], x.v[1]);
+    } else {
+      base[lane] = x.v[0];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < LPL; ++j)
+      if (lane * LPL + j < D) base[lane * LPL + j] = x.v[j];
+  }
+}
+
+// Pipelined chain sweep.  Every input stream of a pass (unaries, cost, backward
+// messages, edge weights) runs P nodes ahead of the node being processed: the
+// main loop handles blocks of P nodes, unrolled, so ring slot j is a fixed set
+// of registers and the load for node s + P is issued right after slot s is read
+// (clamped to the last node, so it never leaves the chain).  The last partial
+// block is an unrolled tail that issues no loads.
+template <int LPL, int MODE, int P, bool FULL>
 __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
+  constexpr bool kCols = MODE == kColsMinorant;
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
   const int n = a.n, D = a.D;
   const long pbase = (long)c * a.pcs;
+  const long nstride = a.pns * (long)D;  // floats between consecutive nodes
+  const float* __restrict__ Ab = a.A + pbase * D;
+  float* __restrict__ Aw = a.A + pbase * D;
+  const float* __restrict__ Cb = kCols ? a.cost + pbase * D : nullptr;
+  const float* __restrict__ wb = a.w + pbase;
   float* __restrict__ msg = a.msg + (long)c * n * D;
   bool valid[LPL];
 #pragma unroll
-  for (int j = 0; j < LPL; ++j) valid[j] = lane * LPL + j < D;
+  for (int j = 0; j < LPL; ++j) valid[j] = FULL || lane * LPL + j < D;
 
-  auto theta_at = [&](int i, Vec<LPL>& th, Vec<LPL>& av) {
-    const long off = (pbase + (long)i * a.pns) * D;
-    load_node<LPL>(a.A + off, lane, D, av, kInfF);
-    if constexpr (MODE == kColsMinorant) {
-      Vec<LPL> cv;
-      load_node<LPL>(a.cost + off, lane, D, cv, kInfF);
+  auto theta_of = [&](const Vec<LPL>& av, const Vec<LPL>& cv) {
+    Vec<LPL> th;
 #pragma unroll
-      for (int j = 0; j < LPL; ++j) th.v[j] = valid[j] ? cv.v[j] - av.v[j] : kInfF;
-    } else {
-#pragma unroll
-      for (int j = 0; j < LPL; ++j) th.v[j] = valid[j] ? av.v[j] : kInfF;
-    }
+    for (int j = 0; j < LPL; ++j) th.v[j] = valid[j] ? (kCols ? cv.v[j] - av.v[j] : av.v[j]) : kInfF;
+    return th;
   };
-  auto weight_at = [&](int i) { return a.w[pbase + (long)i * a.pns]; };
 
-  // ---- backward pass: Bh[n-1] = 0, Bh[i] = minconv(theta[i+1] + Bh[i+1]) - min(...)
+  // The chain's edge weights go to LDS once; each step reads its weight there.
+  float* w_lds = reinterpret_cast<float*>(b_lds + n);
+  for (int i = lane; i < n - 1; i += 64) w_lds[i] = wb[(long)i * a.pns];
+  __syncthreads();
+
+  Vec<LPL> ra[P], rc[P], rb[P];
+  float rw[P];
+
+  // ---- backward pass: step s handles i = n-2-s: Bh[i] = minconv(theta[i+1] + Bh[i+1]) - min(...)
+  const int nb = n - 1;
+  auto load_b = [&](int s, Vec<LPL>& av, Vec<LPL>& cv, float& w) {
+    const int node = s < nb ? n - 1 - s : 1;  // clamped: beyond the chain re-read node 1
+    av = ld_node<LPL, FULL>(Ab + node * nstride, lane, D, kInfF);
+    if constexpr (kCols) cv = ld_node<LPL, FULL>(Cb + node * nstride, lane, D, kInfF);
+    (void)w;
+  };
   Vec<LPL> bh;
 #pragma unroll
   for (int j = 0; j < LPL; ++j) bh.v[j] = 0.f;
-  store_node<LPL>(msg + (long)(n - 1) * D, lane, D, bh);
+  st_node<LPL, FULL>(msg + (long)(n - 1) * D, lane, D, bh);
   double bsum = 0.0;
-  if (lane == 0) b_lds[n - 1] = 0.0;
-  Vec<LPL> th_next, a_tmp;
-  if (n >= 2) theta_at(n - 1, th_next, a_tmp);
-  for (int i = n - 2; i >= 0; --i) {
-    Vec<LPL> th_cur, a_cur;
-    if (i >= 0) theta_at(i, th_cur, a_cur);  // prefetch theta[i] for the next step
-    const float wi = weight_at(i);
+  b_lds[n - 1] = 0.0;
+  auto step_b = [&](int s, const Vec<LPL>& th, float) {
+    const int i = n - 2 - s;
+    const float wi = w_lds[i];
     Vec<LPL> U;
 #pragma unroll
-    for (int j = 0; j < LPL; ++j) U.v[j] = th_next.v[j] + bh.v[j];
+    for (int k = 0; k < LPL; ++k) U.v[k] = th.v[k] + bh.v[k];
     const float u = wave_min(local_min(U));
     const Vec<LPL> m = minconv(U, u, wi, a.L1, a.L2, a.delta);
 #pragma unroll
-    for (int j = 0; j < LPL; ++j) bh.v[j] = valid[j] ? m.v[j] - u : 0.f;
-    store_node<LPL>(msg + (long)i * D, lane, D, bh);
+    for (int k = 0; k < LPL; ++k) bh.v[k] = valid[k] ? m.v[k] - u : 0.f;
+    st_node<LPL, FULL>(msg + (long)i * D, lane, D, bh);
     bsum += (double)u;
-    if (lane == 0) b_lds[i] = bsum;
-    th_next = th_cur;
+    b_lds[i] = bsum;  // every lane writes the same value
+  };
+  if (nb > 0) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) load_b(j, ra[j], rc[j], rw[j]);
   }
-  // Here th_next holds theta[0] (or theta[n-1] == theta[0] when n == 1), bh holds Bh[0].
-  Vec<LPL> th0, a0;
-  if (n >= 2) {
-    th0 = th_next;
-  } else {
-    theta_at(0, th0, a0);
+  int s0 = 0;
+  for (; s0 + P <= nb; s0 += P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const Vec<LPL> th = theta_of(ra[j], rc[j]);
+      const float wi = rw[j];
+      // Keep "read slot j" before "refill slot j": the refill then reuses the
+      // slot's registers and the ring needs no copies at the loop back-edge.
+      __builtin_amdgcn_sched_barrier(0);
+      load_b(s0 + j + P, ra[j], rc[j], rw[j]);
+      step_b(s0 + j, th, wi);
+    }
   }
+#pragma unroll
+  for (int j = 0; j < P; ++j)
+    if (s0 + j < nb) step_b(s0 + j, theta_of(ra[j], rc[j]), rw[j]);
+
+  // bh holds Bh[0]; theta[0] is read directly.
+  const Vec<LPL> a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
+  Vec<LPL> c0;
+  if constexpr (kCols) c0 = ld_node<LPL, FULL>(Cb, lane, D, kInfF);
+  const Vec<LPL> th0 = theta_of(a0, c0);
   __syncthreads();
-  const double b0 = b_lds[0];
   Vec<LPL> M0;
 #pragma unroll
-  for (int j = 0; j < LPL; ++j) M0.v[j] = th0.v[j] + bh.v[j];
-  const double Lambda = b0 + (double)wave_min(local_min(M0));
+  for (int k = 0; k < LPL; ++k) M0.v[k] = th0.v[k] + bh.v[k];
+  const double Lambda = bsum + (double)wave_min(local_min(M0));
 
   if constexpr (MODE == kRowsDecode) {
     // Forward min-sum F, label = argmin(F + Bh) (lowest label on ties).
+    auto load_d = [&](int i, Vec<LPL>& bv, Vec<LPL>& av, float& w) {
+      const int node = i < n ? i : n - 1;
+      bv = ld_node<LPL, FULL>(msg + (long)node * D, lane, D, 0.f);
+      av = ld_node<LPL, FULL>(Ab + node * nstride, lane, D, kInfF);
+      (void)w;
+    };
     Vec<LPL> F = th0;
-    for (int i = 0; i < n; ++i) {
+    auto step_d = [&](int i, const Vec<LPL>& b, const Vec<LPL>& an, float) {
       const long p = pbase + (long)i * a.pns;
       if (i > 0) {
+        const float wv = w_lds[i - 1];
         const float f = wave_min(local_min(F));
-        const float wi = weight_at(i - 1);
-        const Vec<LPL> m = minconv(F, f, wi, a.L1, a.L2, a.delta);
-        Vec<LPL> th, av;
-        theta_at(i, th, av);
+        const Vec<LPL> m = minconv(F, f, wv, a.L1, a.L2, a.delta);
 #pragma unroll
-        for (int j = 0; j < LPL; ++j) F.v[j] = valid[j] ? th.v[j] + m.v[j] - f : kInfF;
+        for (int k = 0; k < LPL; ++k) F.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
       }
-      Vec<LPL> b;
-      load_node<LPL>(msg + (long)i * D, lane, D, b, 0.f);
       float best = kInfF;
       int bj = 0;
 #pragma unroll
-      for (int j = 0; j < LPL; ++j) {
-        const float mm = valid[j] ? F.v[j] + b.v[j] : kInfF;
+      for (int k = 0; k < LPL; ++k) {
+        const float mm = valid[k] ? F.v[k] + b.v[k] : kInfF;
         if (mm < best) {
           best = mm;
-          bj = j;
+          bj = k;
         }
       }
       const float gbest = wave_min(best);
@@ -259,67 +353,101 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       if (a.gap) {
         float second = kInfF;
 #pragma unroll
-        for (int j = 0; j < LPL; ++j) {
-          const float mm = valid[j] ? F.v[j] + b.v[j] : kInfF;
-          if (lane * LPL + j != label) second = fminf(second, mm);
+        for (int k = 0; k < LPL; ++k) {
+          const float mm = valid[k] ? F.v[k] + b.v[k] : kInfF;
+          if (lane * LPL + k != label) second = fminf(second, mm);
         }
         const float gsecond = wave_min(second);
         if (lane == 0) a.gap[p] = gsecond - gbest;
       }
       if (lane == 0) a.labels[p] = label;
+    };
+#pragma unroll
+    for (int j = 0; j < P; ++j) load_d(j, rb[j], ra[j], rw[j]);
+    int i0 = 0;
+    for (; i0 + P <= n; i0 += P) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const Vec<LPL> b = rb[j];
+        const Vec<LPL> an = ra[j];
+        const float wv = rw[j];
+        __builtin_amdgcn_sched_barrier(0);
+        load_d(i0 + j + P, rb[j], ra[j], rw[j]);
+        step_d(i0 + j, b, an, wv);
+      }
     }
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (i0 + j < n) step_d(i0 + j, rb[j], ra[j], rw[j]);
     if (lane == 0) a.lam[c] = Lambda;
-    return;
   } else {
     // Forward pass: keep g_i = gamma (R + Bh) + excess, pass V = R - gamma (R + Bh) on.
+    auto load_f = [&](int i, Vec<LPL>& bv, Vec<LPL>& av, Vec<LPL>& cv, float& w) {
+      const int node = i < nb ? i : nb - 1;
+      bv = ld_node<LPL, FULL>(msg + (long)node * D, lane, D, 0.f);
+      av = ld_node<LPL, FULL>(Ab + (node + 1) * nstride, lane, D, kInfF);
+      if constexpr (kCols) cv = ld_node<LPL, FULL>(Cb + (node + 1) * nstride, lane, D, kInfF);
+      (void)w;
+    };
     Vec<LPL> R = th0;
+    Vec<LPL> a_cur = a0;
     double r = 0.0, kept = 0.0;
-    Vec<LPL> a_cur;
-    {
-      Vec<LPL> tdummy;
-      theta_at(0, tdummy, a_cur);
-    }
-    for (int i = 0; i + 1 < n; ++i) {
-      const long off = (pbase + (long)i * a.pns) * D;
-      Vec<LPL> b;
-      load_node<LPL>(msg + (long)i * D, lane, D, b, 0.f);
-      Vec<LPL> th_n, a_n;
-      theta_at(i + 1, th_n, a_n);
-      const float wi = weight_at(i);
+    auto step_f = [&](int i, const Vec<LPL>& b, const Vec<LPL>& an, const Vec<LPL>& cn, float) {
       const double bi = b_lds[i];
+      const float wi = w_lds[i];
       Vec<LPL> gt, V;
 #pragma unroll
-      for (int j = 0; j < LPL; ++j) {
-        gt.v[j] = valid[j] ? kGamma * (R.v[j] + b.v[j]) : 0.f;
-        V.v[j] = valid[j] ? R.v[j] - gt.v[j] : kInfF;
+      for (int k = 0; k < LPL; ++k) {
+        gt.v[k] = valid[k] ? kGamma * (R.v[k] + b.v[k]) : 0.f;
+        V.v[k] = valid[k] ? R.v[k] - gt.v[k] : kInfF;
       }
       kept += kGammaD * (r + bi);
       const float v = wave_min(local_min(V));
       const Vec<LPL> m = minconv(V, v, wi, a.L1, a.L2, a.delta);
       Vec<LPL> mm;
 #pragma unroll
-      for (int j = 0; j < LPL; ++j) mm.v[j] = valid[j] ? m.v[j] : -kInfF;
+      for (int k = 0; k < LPL; ++k) mm.v[k] = valid[k] ? m.v[k] : -kInfF;
       const Vec<LPL> nmax = near_max(mm, a.delta);
       const float wl1 = wi * a.L1;
+      const Vec<LPL> thn = theta_of(an, cn);
       Vec<LPL> out;
 #pragma unroll
-      for (int j = 0; j < LPL; ++j) {
-        const float e = valid[j] ? V.v[j] - fmaxf(m.v[j], nmax.v[j] - wl1) : 0.f;
-        const float g = gt.v[j] + e;
-        out.v[j] = MODE == kRowsMinorant ? a_cur.v[j] - g : a_cur.v[j] + g;
-        R.v[j] = valid[j] ? th_n.v[j] + m.v[j] - v : kInfF;
+      for (int k = 0; k < LPL; ++k) {
+        const float e = valid[k] ? V.v[k] - fmaxf(m.v[k], nmax.v[k] - wl1) : 0.f;
+        const float g = gt.v[k] + e;
+        out.v[k] = MODE == kRowsMinorant ? a_cur.v[k] - g : a_cur.v[k] + g;
+        R.v[k] = valid[k] ? thn.v[k] + m.v[k] - v : kInfF;
       }
-      store_node<LPL>(a.A + off, lane, D, out);
+      st_node<LPL, FULL>(Aw + i * nstride, lane, D, out);
       r = (double)v + (1.0 - kGammaD) * r - kGammaD * bi;
-      a_cur = a_n;
+      a_cur = an;
+    };
+    if (nb > 0) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) load_f(j, rb[j], ra[j], rc[j], rw[j]);
     }
+    int i0 = 0;
+    for (; i0 + P <= nb; i0 += P) {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const Vec<LPL> b = rb[j];
+        const Vec<LPL> an = ra[j];
+        const Vec<LPL> cn = rc[j];
+        const float wi = rw[j];
+        __builtin_amdgcn_sched_barrier(0);
+        load_f(i0 + j + P, rb[j], ra[j], rc[j], rw[j]);
+        step_f(i0 + j, b, an, cn, wi);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (i0 + j < nb) step_f(i0 + j, rb[j], ra[j], rc[j], rw[j]);
     {
-      const long off = (pbase + (long)(n - 1) * a.pns) * D;
       Vec<LPL> out;
 #pragma unroll
-      for (int j = 0; j < LPL; ++j)
-        out.v[j] = MODE == kRowsMinorant ? a_cur.v[j] - R.v[j] : a_cur.v[j] + R.v[j];
-      store_node<LPL>(a.A + off, lane, D, out);
+      for (int k = 0; k < LPL; ++k)
+        out.v[k] = MODE == kRowsMinorant ? a_cur.v[k] - R.v[k] : a_cur.v[k] + R.v[k];
+      st_node<LPL, FULL>(Aw + (n - 1) * nstride, lane, D, out);
       kept += r;
     }
     if (lane == 0) {
@@ -443,16 +571,20 @@ __global__ void k_sum_pairs(const double* __restrict__ part, int nb, double* __r
   }
 }
 
+template <int MODE, int LPL, int P>
+static void launch_lpl(ChainArgs& a, size_t lds, hipStream_t st) {
+  if (a.D == 64 * LPL)
+    k_chain<LPL, MODE, P, true><<<a.n_chains, 64, lds, st>>>(a);
+  else
+    k_chain<LPL, MODE, P, false><<<a.n_chains, 64, lds, st>>>(a);
+}
+
 template <int MODE>
 static int launch_chain(ChainArgs& a, hipStream_t st) {
-  const size_t lds = sizeof(double) * (size_t)a.n;
-  if (a.D <= 64) {
-    k_chain<1, MODE><<<a.n_chains, 64, lds, st>>>(a);
-  } else if (a.D <= 128) {
-    k_chain<2, MODE><<<a.n_chains, 64, lds, st>>>(a);
-  } else {
-    k_chain<4, MODE><<<a.n_chains, 64, lds, st>>>(a);
-  }
+  const size_t lds = (sizeof(double) + sizeof(float)) * (size_t)a.n;
+  if (a.D <= 64) launch_lpl<MODE, 1, 32>(a, lds, st);
+  else if (a.D <= 128) launch_lpl<MODE, 2, 16>(a, lds, st);
+  else launch_lpl<MODE, 4, 8>(a, lds, st);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }
@@ -461,51 +593,76 @@ static int launch_chain(ChainArgs& a, hipStream_t st) {
 
 using namespace ccs;
 
+static int check_crf_args(int H, int W, int D, float L1, float L2, int delta) {
+  CCS_CHECK(H >= 1 && W >= 1 && D >= 2 && D <= 256, CCS_ERR_ARG,
+            "slackprop: need H, W >= 1 and 2 <= D <= 256 (got H=%d W=%d D=%d)", H, W, D);
+  CCS_CHECK(H <= 8192 && W <= 8192, CCS_ERR_UNSUPPORTED, "slackprop: chains longer than 8192");
+  CCS_CHECK(delta >= 1 && delta <= 64, CCS_ERR_ARG, "slackprop: delta must be in [1, 64] (got %d)", delta);
+  CCS_CHECK(L1 >= 0.f && L2 >= L1, CCS_ERR_ARG,
+            "slackprop: the min-sum form needs 0 <= L1 <= L2 (got L1=%g L2=%g)", L1, L2);
+  return CCS_OK;
+}
+
+extern "C" int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, const float* wx, const float* wy,
+                                   int H, int W, int D, float L1, float L2, int delta, int direction, int* labels,
+                                   double* chain_sum, double* kept_sum) {
+  CCS_CHECK(h && A, CCS_ERR_ARG, "ccs_slackprop_sweep: NULL argument");
+  int rc = check_crf_args(H, W, D, L1, L2, delta);
+  if (rc) return rc;
+  CCS_CHECK(direction >= 0 && direction <= 2, CCS_ERR_ARG, "ccs_slackprop_sweep: direction must be 0, 1 or 2");
+  CCS_CHECK(direction != 0 || wx, CCS_ERR_ARG, "ccs_slackprop_sweep: rows need wx");
+  CCS_CHECK(direction != 1 || (wy && cost), CCS_ERR_ARG, "ccs_slackprop_sweep: columns need cost and wy");
+  CCS_CHECK(direction != 2 || (wx && labels), CCS_ERR_ARG, "ccs_slackprop_sweep: decode needs wx and labels");
+  const size_t vol = (size_t)H * W * D;
+  if ((rc = h->crf_msg.reserve(vol * sizeof(float)))) return rc;
+  const int nmax = H > W ? H : W;
+  if ((rc = h->crf_small.reserve(sizeof(double) * 2 * (size_t)nmax))) return rc;
+  double* lam = (double*)h->crf_small.ptr;
+  double* kept = lam + nmax;
+  ChainArgs a{};
+  a.cost = cost; a.A = A; a.msg = (float*)h->crf_msg.ptr; a.lam = lam; a.kept = kept;
+  a.D = D; a.delta = delta; a.L1 = L1; a.L2 = L2;
+  if (direction == 1) {
+    a.w = wy; a.pcs = 1; a.pns = W; a.n_chains = W; a.n = H;
+  } else {
+    a.w = wx; a.pcs = W; a.pns = 1; a.n_chains = H; a.n = W;
+  }
+  if (direction == 0) rc = launch_chain<kRowsMinorant>(a, h->stream);
+  else if (direction == 1) rc = launch_chain<kColsMinorant>(a, h->stream);
+  else {
+    a.labels = labels;
+    rc = launch_chain<kRowsDecode>(a, h->stream);
+  }
+  if (rc) return rc;
+  if (chain_sum) k_sum_f64<<<1, 256, 0, h->stream>>>(lam, a.n_chains, chain_sum);
+  if (kept_sum && direction != 2) k_sum_f64<<<1, 256, 0, h->stream>>>(kept, a.n_chains, kept_sum);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
 extern "C" int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H, int W,
                              int D, float L1, float L2, int delta, int iters, int* labels, float* unaries_out,
                              double* sums, double* lower_bound) {
   CCS_CHECK(h && cost && wx && wy && labels, CCS_ERR_ARG, "ccs_slackprop: NULL argument");
-  CCS_CHECK(H >= 1 && W >= 1 && D >= 2 && D <= 256, CCS_ERR_ARG,
-            "ccs_slackprop: need H, W >= 1 and 2 <= D <= 256 (got H=%d W=%d D=%d)", H, W, D);
-  CCS_CHECK(H <= 8192 && W <= 8192, CCS_ERR_UNSUPPORTED, "ccs_slackprop: chains longer than 8192");
+  int rc = check_crf_args(H, W, D, L1, L2, delta);
+  if (rc) return rc;
   CCS_CHECK(iters >= 0 && iters <= 10000, CCS_ERR_ARG, "ccs_slackprop: bad iteration count %d", iters);
-  CCS_CHECK(delta >= 1 && delta <= 64, CCS_ERR_ARG, "ccs_slackprop: delta must be in [1, 64] (got %d)", delta);
-  CCS_CHECK(L1 >= 0.f && L2 >= L1, CCS_ERR_ARG,
-            "ccs_slackprop: the min-sum form needs 0 <= L1 <= L2 (got L1=%g L2=%g)", L1, L2);
   const size_t vol = (size_t)H * W * D;
-  int rc;
   if ((rc = h->crf_a.reserve(vol * sizeof(float)))) return rc;
-  if ((rc = h->crf_msg.reserve(vol * sizeof(float)))) return rc;
-  const int nmax = H > W ? H : W;
-  const int nsums = 4 * iters + 2;
-  if ((rc = h->crf_small.reserve(sizeof(double) * (2 * (size_t)nmax + nsums)))) return rc;
+  if ((rc = h->crf_sums.reserve(sizeof(double) * (4 * (size_t)iters + 2)))) return rc;
   float* A = (float*)h->crf_a.ptr;
-  double* lam = (double*)h->crf_small.ptr;
-  double* kept = lam + nmax;
-  double* S = kept + nmax;            // S_0 .. S_{2 iters}
-  double* K = S + 2 * iters + 1;      // K_0 .. K_{2 iters - 1}
+  double* S = (double*)h->crf_sums.ptr;  // S_0 .. S_{2 iters}
+  double* K = S + 2 * iters + 1;         // K_0 .. K_{2 iters - 1}
   CCS_HIP(hipMemcpyAsync(A, cost, vol * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
-
-  ChainArgs rows{};
-  rows.cost = cost; rows.A = A; rows.w = wx; rows.msg = (float*)h->crf_msg.ptr;
-  rows.lam = lam; rows.kept = kept; rows.pcs = W; rows.pns = 1;
-  rows.n_chains = H; rows.n = W; rows.D = D; rows.delta = delta; rows.L1 = L1; rows.L2 = L2;
-  ChainArgs cols = rows;
-  cols.w = wy; cols.pcs = 1; cols.pns = W; cols.n_chains = W; cols.n = H;
-
   for (int it = 0; it < iters; ++it) {
-    if ((rc = launch_chain<kRowsMinorant>(rows, h->stream))) return rc;
-    k_sum_f64<<<1, 256, 0, h->stream>>>(lam, H, S + 2 * it);
-    k_sum_f64<<<1, 256, 0, h->stream>>>(kept, H, K + 2 * it);
-    if ((rc = launch_chain<kColsMinorant>(cols, h->stream))) return rc;
-    k_sum_f64<<<1, 256, 0, h->stream>>>(lam, W, S + 2 * it + 1);
-    k_sum_f64<<<1, 256, 0, h->stream>>>(kept, W, K + 2 * it + 1);
+    if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, 0, nullptr, S + 2 * it, K + 2 * it)))
+      return rc;
+    if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, 1, nullptr, S + 2 * it + 1,
+                                  K + 2 * it + 1)))
+      return rc;
   }
-  ChainArgs dec = rows;
-  dec.labels = labels;
-  dec.gap = nullptr;
-  if ((rc = launch_chain<kRowsDecode>(dec, h->stream))) return rc;
-  k_sum_f64<<<1, 256, 0, h->stream>>>(lam, H, S + 2 * iters);
+  if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, 2, labels, S + 2 * iters, nullptr)))
+    return rc;
   if (lower_bound) k_bound<<<1, 1, 0, h->stream>>>(S, iters, lower_bound);
   if (sums)
     CCS_HIP(hipMemcpyAsync(sums, S, sizeof(double) * (4 * (size_t)iters + 1), hipMemcpyDeviceToDevice, h->stream));

@@ -75,6 +75,7 @@ extern "C" int ccs_destroy(ccs_handle_t h) {
   h->crf_a.release();
   h->crf_msg.release();
   h->crf_small.release();
+  h->crf_sums.release();
   h->misc.release();
   h->misc2.release();
   delete h;

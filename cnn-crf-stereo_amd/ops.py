@@ -181,6 +181,13 @@ def slackprop(h, cost, wx, wy, L1, L2, delta, iters, want_unaries=False):
     return out
 
 
+def slackprop_sweep(h, cost, A, wx, wy, L1, L2, delta, direction, labels=None, chain_sum=None, kept_sum=None):
+    """One half-step (0 rows, 1 columns, 2 decode) on the caller's row unaries A."""
+    H, W, D = cost.shape
+    call("ccs_slackprop_sweep", h.h, ptr(cost), ptr(A), ptr(wx), ptr(wy), H, W, D, float(L1), float(L2),
+         int(delta), int(direction), ptr(labels), ptr(chain_sum), ptr(kept_sum))
+
+
 def slackprop_decode_gap(h, unaries, wx, L1, L2, delta):
     H, W, D = unaries.shape
     labels = torch.empty((H, W), dtype=torch.int32, device="cuda")

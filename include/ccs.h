@@ -174,6 +174,16 @@ int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx, const floa
                   int W, int D, float L1, float L2, int delta, int iters, int* labels,
                   float* unaries_out, double* sums, double* lower_bound);
 
+/* One half-step of ccs_slackprop on caller-held row unaries A (H x W x D, set to
+ * a copy of cost before the first call): direction 0 sweeps the rows
+ * (A -= minorant), 1 the columns (A += minorant, reads cost), 2 decodes the rows
+ * into labels.  chain_sum / kept_sum (optional device doubles) receive S_j / K_j.
+ * The steps of slack_prop_2D_alg::execute (stereo/stereo.cpp:153), exposed for
+ * callers that schedule or time the sweeps themselves. */
+int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, const float* wx, const float* wy,
+                        int H, int W, int D, float L1, float L2, int delta, int direction,
+                        int* labels, double* chain_sum, double* kept_sum);
+
 /* CRF energy of a labelling, accumulated in float64 (device double). */
 int ccs_crf_energy(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H,
                    int W, int D, const int* labels, float L1, float L2, int delta,
