@@ -70,6 +70,8 @@ inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 typedef __bf16 bf16;
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
+// Native 16-byte vector (unlike HIP's uint4 struct it stays in registers when copied).
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
 
 __device__ __forceinline__ float bf16_to_f32(bf16 v) { return (float)v; }
 __device__ __forceinline__ bf16 f32_to_bf16(float v) { return (bf16)v; }

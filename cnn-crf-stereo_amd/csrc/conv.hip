@@ -25,9 +25,12 @@ namespace ccs {
 // ---------------------------------------------------------------------------
 // Weight packing
 
-// bf16 layout: [q][n][RPB] with q = chunk of 32 reduction indices, n < 16*NT,
-// RPB = 40 elements per row (80 bytes: conflict-free 16-byte LDS reads).
-constexpr int RPB = 40;
+// bf16 layout: [q][n][32] -- q = chunk of 32 reduction indices, n < 16*NT.  The
+// four 8-element groups of row n are stored XOR-swizzled by (n >> 1) & 3, so the
+// 16-byte LDS reads of eight consecutive rows land in eight distinct bank
+// groups without padding the rows.
+constexpr int RPB = 32;
+__host__ __device__ __forceinline__ int bswz(int n, int g) { return g ^ ((n >> 1) & 3); }
 // f32 layout: [q][kk < 4][n] (n contiguous).
 
 __device__ __forceinline__ float flipped_weight(const float* __restrict__ w, int n, int c, int tap, int K, int C,
@@ -46,13 +49,9 @@ __global__ void k_pack_bf16(const float* __restrict__ w, int K, int C, int Cs, i
   const int kk = (int)(idx % RPB);
   const int n = (int)((idx / RPB) % NP);
   const int q = (int)(idx / ((long)RPB * NP));
-  float v = 0.f;
-  if (kk < 32) {
-    const int k = q * 32 + kk;
-    const int tap = k / Cs, c = k % Cs;
-    v = flipped_weight(w, n, c, tap, K, C, KH, KW);
-  }
-  out[idx] = f32_to_bf16(v);
+  const int k = q * 32 + bswz(n, kk / 8) * 8 + kk % 8;  // the swizzle is an involution
+  const int tap = k / Cs, c = k % Cs;
+  out[idx] = f32_to_bf16(flipped_weight(w, n, c, tap, K, C, KH, KW));
 }
 
 __global__ void k_pack_f32(const float* __restrict__ w, int K, int C, int Cs, int KH, int KW, int NQ, int NP,
@@ -70,126 +69,164 @@ __global__ void k_pack_f32(const float* __restrict__ w, int K, int C, int Cs, in
 // ---------------------------------------------------------------------------
 // BF16 MFMA implicit GEMM
 
-constexpr int kTH = 4;    // output rows per workgroup (one per wave)
-constexpr int kTW = 64;   // output pixels per row
+constexpr int kTH = 4;    // output rows per tile (one per wave)
 constexpr int kLdsBytes = 160 * 1024;
 
-template <int KH, int KW, int CG, int NT>
+// TW output pixels per row; the input tile plus one weight group fit in
+// LDS_BUDGET bytes so that 160 KB / LDS_BUDGET workgroups share a CU.
+template <int KH, int KW, int CG, int NT, int TW, int LDS_BUDGET>
 struct Bf16Cfg {
   static constexpr int Cs = CG * 8;
+  static constexpr int MT = TW / 16;              // M-tiles per wave
   static constexpr int taps = KH * KW;
   static constexpr int NQ = (taps * CG + 3) / 4;  // 32-wide reduction chunks
   static constexpr int NP = 16 * NT;
-  static constexpr int AR = kTH + KH - 1, AC = kTW + KW - 1;
-  static constexpr int a_bytes = (AR * AC * Cs * 2 + 15) / 16 * 16;
+  static constexpr int AR = kTH + KH - 1, AC = TW + KW - 1;
+  static constexpr int a_vec = AR * AC * CG;      // 16-byte vectors of the input tile
+  static constexpr int a_bytes = a_vec * 16;
   static constexpr int chunk_bytes = NP * RPB * 2;
-  static constexpr int QG_max = (kLdsBytes - a_bytes) / chunk_bytes;
-  static constexpr int QG = QG_max < NQ ? QG_max : NQ;
-  static constexpr int out_bytes = kTH * kTW * Cs * 2;
-  static_assert(QG >= 1, "conv tile does not fit in LDS");
+  static constexpr int QG_max = (LDS_BUDGET - a_bytes) / chunk_bytes;
+  static constexpr int NG = (NQ + QG_max - 1) / QG_max;  // weight groups per tile
+  static constexpr int QG = (NQ + NG - 1) / NG;          // chunks per group (balanced)
+  static constexpr int B_VPT = (QG * chunk_bytes / 16 + 255) / 256;
+  static constexpr int lds_bytes = a_bytes + QG * chunk_bytes;
+  static constexpr int out_bytes = kTH * TW * Cs * 2;
+  static_assert(QG_max >= 1, "conv tile does not fit in LDS");
   static_assert(out_bytes <= a_bytes, "output staging must fit in the input tile");
 };
 
-template <int KH, int KW, int CG, int NT>
+// One workgroup = 4 waves = a 4-row x TW-pixel output tile for all output
+// channels.  The (4+KH-1) x (TW+KW-1) input halo tile is staged in LDS once;
+// the packed weights stream through LDS in groups, the next group travelling
+// in registers while the current one feeds the MFMAs.  Inside a group the A/B
+// fragments of chunk q+1 are read from LDS while chunk q's MFMAs issue.
+template <int KH, int KW, int CG, int NT, int TW, int LDS_BUDGET>
 __global__ __launch_bounds__(256) void k_conv_bf16(const bf16* __restrict__ x, int Hin, int Win,
                                                    const bf16* __restrict__ wpk, const float* __restrict__ bias,
                                                    int K, int act, bf16* __restrict__ y, int Ho, int Wo) {
-  using Cfg = Bf16Cfg<KH, KW, CG, NT>;
-  constexpr int Cs = Cfg::Cs;
+  using Cfg = Bf16Cfg<KH, KW, CG, NT, TW, LDS_BUDGET>;
+  constexpr int Cs = Cfg::Cs, MT = Cfg::MT;
   extern __shared__ __align__(16) unsigned char lds[];
   bf16* As = reinterpret_cast<bf16*>(lds);
   bf16* Bs = reinterpret_cast<bf16*>(lds + Cfg::a_bytes);
-
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int w0 = blockIdx.x * kTW, h0 = blockIdx.y * kTH;
+  const int lrow = lane & 15, lgrp = lane >> 4;
+  const int w0 = blockIdx.x * TW, h0 = blockIdx.y * kTH;
 
-  // Stage the input halo tile: AR rows of AC pixels, 16-byte vectors (8 channels).
+  // Weight group g: vectors [g*QG*chunk/16, ...) of the packed weights.
+#define CCS_B_NVEC(g) (min(Cfg::QG, Cfg::NQ - (g) * Cfg::QG) * (Cfg::chunk_bytes / 16))
+#define CCS_B_LOAD(g)                                                                                  \
+  {                                                                                                    \
+    const u32x4* src_ = reinterpret_cast<const u32x4*>(wpk) + (long)(g) * Cfg::QG * (Cfg::chunk_bytes / 16); \
+    const int nv_ = CCS_B_NVEC(g);                                                                     \
+    _Pragma("unroll") for (int i = 0; i < Cfg::B_VPT; ++i) wr[i] = src_[min(tid + 256 * i, nv_ - 1)]; \
+  }
+#define CCS_B_STORE(g)                                                                                 \
+  {                                                                                                    \
+    const int nv_ = CCS_B_NVEC(g);                                                                     \
+    _Pragma("unroll") for (int i = 0; i < Cfg::B_VPT; ++i) {                                           \
+      const int v_ = tid + 256 * i;                                                                    \
+      if (v_ < nv_) reinterpret_cast<u32x4*>(Bs)[v_] = wr[i];                                          \
+    }                                                                                                  \
+  }
+
+  u32x4 wr[Cfg::B_VPT];
+  CCS_B_LOAD(0);
   {
     constexpr int vec_per_row = Cfg::AC * CG;
-    constexpr int total = Cfg::AR * vec_per_row;
-    for (int v = tid; v < total; v += 256) {
+    for (int v = tid; v < Cfg::a_vec; v += 256) {
       const int r = v / vec_per_row, rem = v % vec_per_row;
       const int px = rem / CG, cg = rem % CG;
       const int hi = h0 + r, wi = w0 + px;
-      uint4 val = make_uint4(0, 0, 0, 0);
-      if (hi < Hin && wi < Win)
-        val = *reinterpret_cast<const uint4*>(x + (((long)hi * Win + wi) * Cs + cg * 8));
-      *reinterpret_cast<uint4*>(As + ((r * Cfg::AC + px) * Cs + cg * 8)) = val;
+      const bool ok = hi < Hin && wi < Win;
+      const u32x4 val = *reinterpret_cast<const u32x4*>(
+          x + (((long)min(hi, Hin - 1) * Win + min(wi, Win - 1)) * Cs + cg * 8));
+      reinterpret_cast<u32x4*>(As)[v] = ok ? val : u32x4{0u, 0u, 0u, 0u};
     }
   }
+  CCS_B_STORE(0);
+  __syncthreads();
 
-  f32x4 acc[4][NT];
+  f32x4 acc[MT][NT];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int m = 0; m < MT; ++m)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  const int lrow = lane & 15, lgrp = lane >> 4;
-  for (int q0 = 0; q0 < Cfg::NQ; q0 += Cfg::QG) {
-    const int nq = (Cfg::NQ - q0) < Cfg::QG ? (Cfg::NQ - q0) : Cfg::QG;
-    __syncthreads();  // previous group fully consumed (and A tile staged on the first pass)
-    {
-      const int nvec = nq * Cfg::chunk_bytes / 16;
-      const uint4* src = reinterpret_cast<const uint4*>(wpk + (long)q0 * Cfg::NP * RPB);
-      uint4* dst = reinterpret_cast<uint4*>(Bs);
-      for (int v = tid; v < nvec; v += 256) dst[v] = src[v];
+  auto load_frags = [&](int q, int ql, bf16x8 (&a)[MT], bf16x8 (&b)[NT]) {
+    const int kg = 4 * q + lgrp;  // 8-element group of the reduction index
+    const bool kvalid = kg < Cfg::taps * CG;
+    const int tap = kvalid ? kg / CG : 0, c0 = kvalid ? (kg % CG) * 8 : 0;
+    const int ti = tap / KW, tj = tap % KW;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(As + (((wave + ti) * Cfg::AC + 16 * m + lrow + tj) * Cs + c0));
+      a[m] = kvalid ? v : bf16x8{};
     }
-    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int n = 16 * t + lrow;
+      b[t] = *reinterpret_cast<const bf16x8*>(Bs + ((ql * Cfg::NP + n) * RPB + 8 * bswz(n, lgrp)));
+    }
+  };
+
+  for (int g = 0; g < Cfg::NG; ++g) {
+    const bool more = g + 1 < Cfg::NG;
+    if (more) CCS_B_LOAD(g + 1);
+    const int q0 = g * Cfg::QG;
+    const int nq = min(Cfg::QG, Cfg::NQ - q0);
+    bf16x8 a[MT], b[NT];
+    load_frags(q0, 0, a, b);
     for (int ql = 0; ql < nq; ++ql) {
-      const int kg = 4 * (q0 + ql) + lgrp;  // 8-element group of the reduction index
-      const int tap = kg / CG, c0 = (kg % CG) * 8;
-      const bool kvalid = tap < Cfg::taps;
-      const int ti = kvalid ? tap / KW : 0, tj = kvalid ? tap % KW : 0;
-      bf16x8 a[4];
+      bf16x8 an[MT], bn[NT];
+      if (ql + 1 < nq) load_frags(q0 + ql + 1, ql + 1, an, bn);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        if (kvalid) {
-          const int px = 16 * m + lrow + tj;
-          a[m] = *reinterpret_cast<const bf16x8*>(As + (((wave + ti) * Cfg::AC + px) * Cs + c0));
-        } else {
-          a[m] = bf16x8{};
-        }
-      }
+      for (int t = 0; t < NT; ++t)
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(Bs + ((ql * Cfg::NP + 16 * t + lrow) * RPB + 8 * lgrp));
+        for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[t], acc[m][t], 0, 0, 0);
 #pragma unroll
-        for (int m = 0; m < 4; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b, acc[m][t], 0, 0, 0);
-      }
+      for (int m = 0; m < MT; ++m) a[m] = an[m];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b[t] = bn[t];
+    }
+    __syncthreads();  // every wave is done with this group (and, after the last one, with the tile)
+    if (more) {
+      CCS_B_STORE(g + 1);
+      __syncthreads();
     }
   }
 
-  // Epilogue: bias + tanh -> bf16 into LDS [row][pixel][Cs], then coalesced stores.
-  __syncthreads();
+  // Epilogue: bias + tanh -> bf16 staged in the input-tile region, then contiguous stores.
   bf16* Os = As;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int n = 16 * t + lrow;
     const float bn = (n < K && bias) ? bias[n] : 0.f;
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int px = 16 * m + 4 * lgrp + e;
         float v = acc[m][t][e] + bn;
         if (act) v = tanhf(v);
-        if (n < Cs) Os[(wave * kTW + px) * Cs + n] = f32_to_bf16(n < K ? v : 0.f);
+        if (n < Cs) Os[(wave * TW + px) * Cs + n] = f32_to_bf16(n < K ? v : 0.f);
       }
   }
   __syncthreads();
-  {
-    constexpr int vec_per_row = kTW * CG;
-    for (int v = tid; v < kTH * vec_per_row; v += 256) {
-      const int r = v / vec_per_row, rem = v % vec_per_row;
-      const int px = rem / CG, cg = rem % CG;
-      const int ho = h0 + r, wo = w0 + px;
-      if (ho < Ho && wo < Wo)
-        *reinterpret_cast<uint4*>(y + (((long)ho * Wo + wo) * Cs + cg * 8)) =
-            *reinterpret_cast<const uint4*>(Os + ((r * kTW + px) * Cs + cg * 8));
-    }
+  constexpr int vec_per_row = TW * CG;
+  for (int v = tid; v < kTH * vec_per_row; v += 256) {
+    const int r = v / vec_per_row, rem = v % vec_per_row;
+    const int px = rem / CG, cg = rem % CG;
+    const int ho = h0 + r, wo = w0 + px;
+    if (ho < Ho && wo < Wo)
+      *reinterpret_cast<u32x4*>(y + (((long)ho * Wo + wo) * Cs + cg * 8)) =
+          *reinterpret_cast<const u32x4*>(Os + ((r * TW + px) * Cs + cg * 8));
   }
 }
+#undef CCS_B_NVEC
+#undef CCS_B_LOAD
+#undef CCS_B_STORE
 
 // ---------------------------------------------------------------------------
 // FP32 MFMA implicit GEMM (v_mfma_f32_16x16x4_f32)
@@ -346,27 +383,103 @@ __global__ void k_conv_generic(const TI* __restrict__ x, int x_nchw, int Hin, in
 }
 
 // ---------------------------------------------------------------------------
+// Image layer: C <= 4 input channels (C x H x W float32, the normalised padded
+// image), 3x3, K <= 128 outputs.  One thread per output pixel computes every
+// output channel; the flipped weights sit in LDS and the block's 256 x Cs output
+// tile is staged in LDS for contiguous stores.
+
+template <int C, typename TO>
+__global__ __launch_bounds__(256) void k_conv_image(const float* __restrict__ x, int Hin, int Win,
+                                                    const float* __restrict__ w, const float* __restrict__ bias,
+                                                    int K, int act, TO* __restrict__ y, int Ho, int Wo) {
+  constexpr int KMAX = 128;
+  __shared__ float ws[C * 9][KMAX];
+  __shared__ float bs[KMAX];
+  extern __shared__ __align__(16) unsigned char lds_out[];
+  TO* os = reinterpret_cast<TO*>(lds_out);
+  const int Cs = (K + 7) / 8 * 8;
+  for (int v = threadIdx.x; v < C * 9 * KMAX; v += 256) {
+    const int k = v % KMAX, t = v / KMAX;  // t = c * 9 + i * 3 + j
+    const int c = t / 9, i = (t % 9) / 3, j = t % 3;
+    ws[t][k] = k < K ? w[(((long)k * C + c) * 3 + (2 - i)) * 3 + (2 - j)] : 0.f;
+  }
+  for (int k = threadIdx.x; k < KMAX; k += 256) bs[k] = (k < K && bias) ? bias[k] : 0.f;
+  __syncthreads();
+  const long p0 = (long)blockIdx.x * 256;
+  const long p = p0 + threadIdx.x;
+  const long P = (long)Ho * Wo;
+  float in[C * 9];
+  if (p < P) {
+    const int ho = (int)(p / Wo), wo = (int)(p % Wo);
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) in[c * 9 + i * 3 + j] = x[((long)c * Hin + ho + i) * Win + wo + j];
+  } else {
+#pragma unroll
+    for (int t = 0; t < C * 9; ++t) in[t] = 0.f;
+  }
+  for (int k0 = 0; k0 < Cs; k0 += 8) {
+    float acc[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] = bs[k0 + e];
+#pragma unroll
+    for (int t = 0; t < C * 9; ++t) {
+      const float4 w0 = *reinterpret_cast<const float4*>(&ws[t][k0]);
+      const float4 w1 = *reinterpret_cast<const float4*>(&ws[t][k0 + 4]);
+      acc[0] += in[t] * w0.x; acc[1] += in[t] * w0.y; acc[2] += in[t] * w0.z; acc[3] += in[t] * w0.w;
+      acc[4] += in[t] * w1.x; acc[5] += in[t] * w1.y; acc[6] += in[t] * w1.z; acc[7] += in[t] * w1.w;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      float v = acc[e];
+      if (act) v = tanhf(v);
+      os[threadIdx.x * Cs + k0 + e] = (TO)(k0 + e < K ? v : 0.f);
+    }
+  }
+  __syncthreads();
+  // The block's pixels are consecutive in memory: copy 16-byte vectors.
+  const long npx = (P - p0) < 256 ? (P - p0) : 256;
+  const int nvec = (int)(npx * Cs * sizeof(TO) / 16);
+  const uint4* src = reinterpret_cast<const uint4*>(os);
+  uint4* dst = reinterpret_cast<uint4*>(y + p0 * Cs);
+  for (int v = threadIdx.x; v < nvec; v += 256) dst[v] = src[v];
+}
+
+// ---------------------------------------------------------------------------
 // Dispatch
 
 struct ConvShape {
   int H, W, C, K, KH, KW;
 };
 
+// Tile width and LDS budget per workgroup of the bf16 kernels (two workgroups per CU).
+#ifndef CCS_BF16_TW
+#define CCS_BF16_TW 32
+#endif
+#ifndef CCS_BF16_LDS
+#define CCS_BF16_LDS (80 * 1024)
+#endif
+constexpr int kBf16TW = CCS_BF16_TW;
+constexpr int kBf16Lds = CCS_BF16_LDS;
+
 template <int KH, int KW, int CG, int NT>
 static bool try_bf16(const ConvShape& s, const void* x, const void* wpk, const float* b, int act, void* y,
                      hipStream_t st) {
   if (s.KH != KH || s.KW != KW || feature_stride(s.C) != CG * 8 || (s.K + 15) / 16 != NT) return false;
-  using Cfg = Bf16Cfg<KH, KW, CG, NT>;
+  using Cfg = Bf16Cfg<KH, KW, CG, NT, kBf16TW, kBf16Lds>;
   const int Ho = s.H - KH + 1, Wo = s.W - KW + 1;
-  const size_t lds = Cfg::a_bytes + (size_t)Cfg::QG * Cfg::chunk_bytes;
   static bool configured = false;
   if (!configured) {
-    hipFuncSetAttribute((const void*)k_conv_bf16<KH, KW, CG, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        kLdsBytes);
+    hipFuncSetAttribute((const void*)k_conv_bf16<KH, KW, CG, NT, kBf16TW, kBf16Lds>,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
     configured = true;
   }
-  k_conv_bf16<KH, KW, CG, NT><<<dim3(ceil_div(Wo, kTW), ceil_div(Ho, kTH)), 256, lds, st>>>(
-      (const bf16*)x, s.H, s.W, (const bf16*)wpk, b, s.K, act, (bf16*)y, Ho, Wo);
+  k_conv_bf16<KH, KW, CG, NT, kBf16TW, kBf16Lds>
+      <<<dim3(ceil_div(Wo, kBf16TW), ceil_div(Ho, kTH)), 256, Cfg::lds_bytes, st>>>(
+          (const bf16*)x, s.H, s.W, (const bf16*)wpk, b, s.K, act, (bf16*)y, Ho, Wo);
   return true;
 }
 
@@ -446,10 +559,32 @@ extern "C" int ccs_conv_pack_weights(ccs_handle_t h, const float* w, int K, int 
   return CCS_OK;
 }
 
+template <int C>
+static void launch_image(const float* x, int H, int W, const float* w, const float* b, int K, int act,
+                         int precision, void* y, hipStream_t st) {
+  const int Ho = H - 2, Wo = W - 2;
+  const long P = (long)Ho * Wo;
+  const int Cs = feature_stride(K);
+  if (precision == CCS_BF16)
+    k_conv_image<C, bf16><<<ceil_div(P, 256), 256, 256 * Cs * 2, st>>>(x, H, W, w, b, K, act, (bf16*)y, Ho, Wo);
+  else
+    k_conv_image<C, float><<<ceil_div(P, 256), 256, 256 * Cs * 4, st>>>(x, H, W, w, b, K, act, (float*)y, Ho, Wo);
+}
+
 static int conv_generic(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C, const float* w,
                         const float* b, int K, int KH, int KW, int act, int precision, void* y) {
   const int Ho = H - KH + 1, Wo = W - KW + 1;
   const int Cs_out = feature_stride(K);
+  if (x_layout == CCS_LAYOUT_NCHW && KH == 3 && KW == 3 && K <= 128 && C <= 4) {
+    switch (C) {
+      case 1: launch_image<1>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
+      case 2: launch_image<2>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
+      case 3: launch_image<3>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
+      default: launch_image<4>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
+    }
+    CCS_LAUNCH_CHECK();
+    return CCS_OK;
+  }
   const long threads = (long)Ho * Wo * (Cs_out / 8);
   const unsigned grid = ceil_div(threads, 256);
   const int nchw = x_layout == CCS_LAYOUT_NCHW;

@@ -123,47 +123,54 @@ __global__ __launch_bounds__(256) void k_corr_volume(CorrArgs a) {
     for (int e = 0; e < 4; ++e) S_w[(4 * lgrp + e) * a.sp + 16 * t + lrow] = acc[t][e];
   __syncthreads();
 
-  // Band entry of pixel r, disparity k: window column r + o_k - o_lo.
+  // Band entry of pixel r, disparity k: S_w[r * sp + r + o_k - o_lo] = band(r)[k * ostep].
   const int D = a.D;
   const bool neg_softmax = a.epilogue == CCS_EPILOGUE_NEG_SOFTMAX;
+  auto band = [&](int r) { return S_w + r * a.sp + r + a.o0 - a.o_lo; };
   if (neg_softmax) {
+    // 4 lanes per pixel: max, then exp(s - max) written back in place, then 1 / sum.
     const int r = lane >> 2, s4 = lane & 3;
+    float* br = band(r);
     float mx = -INFINITY;
-    for (int k = s4; k < D; k += 4) mx = fmaxf(mx, S_w[r * a.sp + r + a.o0 + k * a.ostep - a.o_lo]);
+    for (int k = s4; k < D; k += 4) mx = fmaxf(mx, br[k * a.ostep]);
     mx = fmaxf(mx, __shfl_xor(mx, 1, 64));
     mx = fmaxf(mx, __shfl_xor(mx, 2, 64));
     float sum = 0.f;
-    for (int k = s4; k < D; k += 4) sum += expf(S_w[r * a.sp + r + a.o0 + k * a.ostep - a.o_lo] - mx);
+    for (int k = s4; k < D; k += 4) {
+      const float e = __expf(br[k * a.ostep] - mx);
+      br[k * a.ostep] = e;
+      sum += e;
+    }
     sum += __shfl_xor(sum, 1, 64);
     sum += __shfl_xor(sum, 2, 64);
-    if (s4 == 0) {
-      stats[(wave * 16 + r) * 2] = mx;
-      stats[(wave * 16 + r) * 2 + 1] = 1.f / sum;
-    }
+    if (s4 == 0) stats[wave * 16 + r] = -1.f / sum;
   }
   __syncthreads();
   const long pix0 = (long)h * a.W + w0 + 16 * wave;
   const int npx = min(16, a.W - (w0 + 16 * wave));
   if (npx <= 0) return;
   float* outp = a.out + pix0 * D;
-  const int total = npx * D;
-  auto value = [&](int f) {
-    const int r = f / D, k = f % D;
-    const float s = S_w[r * a.sp + r + a.o0 + k * a.ostep - a.o_lo];
-    if (!neg_softmax) return s;
-    return -expf(s - stats[(wave * 16 + r) * 2]) * stats[(wave * 16 + r) * 2 + 1];
-  };
-  if ((D & 3) == 0) {
-    for (int f = 4 * lane; f < total; f += 256) {
-      float4 v;
-      v.x = value(f);
-      v.y = value(f + 1);
-      v.z = value(f + 2);
-      v.w = value(f + 3);
-      *reinterpret_cast<float4*>(outp + f) = v;
+  const int lp = D >> 2;  // lanes per pixel when every lane stores 4 disparities
+  if ((D & 3) == 0 && lp <= 64 && (64 % lp) == 0) {
+    const int ppi = 64 / lp, rr = lane / lp, k = 4 * (lane % lp);
+    for (int r0 = 0; r0 < npx; r0 += ppi) {
+      const int r = r0 + rr;
+      if (r < npx) {
+        const float* br = band(r);
+        const float sc = neg_softmax ? stats[wave * 16 + r] : 1.f;
+        float4 v;
+        v.x = br[k * a.ostep] * sc;
+        v.y = br[(k + 1) * a.ostep] * sc;
+        v.z = br[(k + 2) * a.ostep] * sc;
+        v.w = br[(k + 3) * a.ostep] * sc;
+        *reinterpret_cast<float4*>(outp + (long)r * D + k) = v;
+      }
     }
   } else {
-    for (int f = lane; f < total; f += 64) outp[f] = value(f);
+    for (int f = lane; f < npx * D; f += 64) {
+      const int r = f / D, k = f % D;
+      outp[f] = band(r)[k * a.ostep] * (neg_softmax ? stats[wave * 16 + r] : 1.f);
+    }
   }
 }
 

@@ -579,12 +579,23 @@ static void launch_lpl(ChainArgs& a, size_t lds, hipStream_t st) {
     k_chain<LPL, MODE, P, false><<<a.n_chains, 64, lds, st>>>(a);
 }
 
+// Prefetch depth (nodes in flight per stream) by labels per lane; the row
+// sweeps have only H chains, so they need a deeper ring than the columns to
+// keep enough bytes in flight.
+#ifndef CCS_CRF_P_ROWS
+#define CCS_CRF_P_ROWS 16
+#endif
+#ifndef CCS_CRF_P_COLS
+#define CCS_CRF_P_COLS 16
+#endif
+
 template <int MODE>
 static int launch_chain(ChainArgs& a, hipStream_t st) {
   const size_t lds = (sizeof(double) + sizeof(float)) * (size_t)a.n;
-  if (a.D <= 64) launch_lpl<MODE, 1, 32>(a, lds, st);
-  else if (a.D <= 128) launch_lpl<MODE, 2, 16>(a, lds, st);
-  else launch_lpl<MODE, 4, 8>(a, lds, st);
+  constexpr int P2 = MODE == kColsMinorant ? CCS_CRF_P_COLS : CCS_CRF_P_ROWS;
+  if (a.D <= 64) launch_lpl<MODE, 1, 2 * P2>(a, lds, st);
+  else if (a.D <= 128) launch_lpl<MODE, 2, P2>(a, lds, st);
+  else launch_lpl<MODE, 4, (P2 / 2 > 4 ? P2 / 2 : 4)>(a, lds, st);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }

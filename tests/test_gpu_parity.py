@@ -93,6 +93,26 @@ def test_conv_layer(handle, precision, C, K, k):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("k", [3, 2])
+def test_conv_layer_many_tiles(handle, precision, k):
+    """More output tiles than compute units: exercises the persistent tile loop
+    (and the prefetch of the next tile's input and weights)."""
+    rng = np.random.default_rng(15 + k)
+    C = K = 100
+    H, W = 4 * 70 + k - 1 + 3, 64 * 4 + k - 1 + 5  # partial tiles on both edges
+    x = rng.uniform(-1, 1, size=(C, H, W)).astype(np.float32)
+    w = glorot(rng, K, C, k, k)
+    b = rng.uniform(-0.1, 0.1, size=K).astype(np.float32)
+    dt = torch.bfloat16 if precision == "bf16" else torch.float32
+    layer = ops.ConvLayer(handle, w, b, precision)
+    y = layer(handle, to_hwc(x, capi.feature_stride(C), dt))
+    if precision == "bf16":
+        x, w = bf16_round(x), bf16_round(w)
+    ref = O.conv2d(x, w, b, True)
+    np.testing.assert_allclose(from_hwc(y, K), ref, atol=8e-3 if precision == "bf16" else 2e-4, rtol=0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
 @pytest.mark.parametrize("C", [1, 3])
 def test_conv_first_layer_nchw(handle, precision, C):
     rng = np.random.default_rng(20 + C)
