@@ -88,8 +88,10 @@ __device__ __forceinline__ float dpp_f(float old, float src) {
 }
 
 template <int CTRL>
-__device__ __forceinline__ float dpp_rot(float x) {  // full masks: every lane has a source
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
+__device__ __forceinline__ float dpp_rot(float x) {
+  // Rotations give every lane a valid source, so bound_ctrl never applies; setting
+  // it lets the compiler fuse the move into the consuming v_min/v_max/v_add.
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
 }
 
 // Minimum over the 64 lanes, left in every lane: rotations inside each row of

@@ -1,0 +1,30 @@
+// Interface between the SlackProp host code and the chain-sweep kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace ccs {
+
+constexpr float kGamma = 0.25f;
+constexpr double kGammaD = 0.25;
+
+enum ChainMode { kRowsMinorant = 0, kColsMinorant = 1, kRowsDecode = 2 };
+
+struct ChainArgs {
+  const float* cost;   // C (column mode)
+  float* A;            // row unaries, updated in place
+  const float* w;      // edge weight of the edge (node i, node i+1), indexed by pixel
+  float* msg;          // backward messages, [chain][node][D]
+  int* labels;         // decode output, indexed by pixel
+  float* gap;          // decode: second best - best min-marginal (optional)
+  double* lam;         // per chain: minimum chain energy
+  double* kept;        // per chain: sum of kept constants
+  long pcs, pns;       // pixel index = chain * pcs + node * pns
+  int n_chains, n, D, delta;
+  float L1, L2;
+};
+
+// Launches one sweep of `mode` over all chains of `a` on `st`.
+int launch_chain(ChainMode mode, ChainArgs& a, hipStream_t st);
+
+}  // namespace ccs
