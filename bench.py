@@ -142,6 +142,21 @@ def time_form(pkg, h, data, spec, params, precision, steps, warmup):
     return ms, L.summary(), out
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch by kernel class, from the committed counter summary
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE passes, scaled by a streaming kernel of
+    known size); empty when no summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s.md" % workload)
+    if not os.path.exists(path):
+        return {}
+    text = open(path).read()
+    start = text.find("```json")
+    if start < 0:
+        return {}
+    body = text[start + len("```json"):text.find("```", start + 7)]
+    return json.loads(body)
+
+
 def bad_pixel_rate(labels, gt, D, border=0):
     import numpy as np
 
@@ -253,6 +268,9 @@ def main():
         roof = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, work_per_launch=per_launch,
                     mean_launch_ms=round(mean_ms, 5))
+    traffic = measured_traffic(args.workload).get(dom)
+    if traffic is not None and args.arch == "all3x3" and args.layers == 7:
+        roof["traffic"] = traffic
     kernels = {}
     for k, v in launches_bf16.items():
         m = v["ms"] / v["launches"]
