@@ -231,9 +231,9 @@ extern "C" int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx,
 extern "C" int ccs_slackprop_decode_gap(ccs_handle_t h, const float* unaries, const float* wx, int H, int W, int D,
                                         float L1, float L2, int delta, int* labels, float* gap) {
   CCS_CHECK(h && unaries && wx && labels && gap, CCS_ERR_ARG, "ccs_slackprop_decode_gap: NULL argument");
-  CCS_CHECK(H >= 1 && W >= 1 && D >= 2 && D <= 256, CCS_ERR_ARG, "ccs_slackprop_decode_gap: bad shape");
+  int rc = check_crf_args(H, W, D, L1, L2, delta);
+  if (rc) return rc;
   const size_t vol = (size_t)H * W * D;
-  int rc;
   if ((rc = h->crf_msg.reserve(vol * sizeof(float)))) return rc;
   if ((rc = h->crf_small.reserve(sizeof(double) * 2 * (size_t)H))) return rc;
   ChainArgs dec{};

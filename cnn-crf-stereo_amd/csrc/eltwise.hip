@@ -1,0 +1,161 @@
+// Stand-alone forms of the operators that the feature towers normally fuse
+// into the convolution epilogue or the correlation epilogue: bias add, tanh,
+// softmax over the channel axis (optionally negated), plus layout moves between
+// the C4 (RGBA) images, planar NCHW tensors and the channel-last feature maps.
+// All kernels stream memory once; 256-thread blocks, grid-stride loops.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace ccs {
+
+constexpr int kThreads = 256;
+
+inline unsigned grid_for(long n) {
+  const long b = (n + kThreads - 1) / kThreads;
+  return (unsigned)(b < 65536 ? (b > 0 ? b : 1) : 65536);
+}
+
+// y[p][c] += b[c] (channel-last, channel stride cs) -- stereo/operators/bias.cpp:58-66.
+__global__ void k_bias_nhwc(float* __restrict__ y, long HW, int C, int cs, const float* __restrict__ b) {
+  const long n = HW * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long p = i / C;
+    const int c = (int)(i - p * C);
+    y[p * cs + c] += b[c];
+  }
+}
+
+// y[c][p] += b[c] (planar).
+__global__ void k_bias_nchw(float* __restrict__ y, long HW, int C, const float* __restrict__ b) {
+  const long n = HW * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] += b[i / HW];
+}
+
+// y = tanh(y) -- stereo/operators/activation.cpp:33-40 (CUDNN_ACTIVATION_TANH).
+__global__ void k_tanh(float* __restrict__ y, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = tanhf(y[i]);
+}
+
+// Softmax over the D channels of each pixel of a channel-last volume, one wave
+// per pixel (stereo/operators/softmax.cpp:30-38, ACCURATE: max subtracted), with
+// the negation of stereo/utils.cu:289-292 folded in when `negate`.
+__global__ __launch_bounds__(256) void k_softmax_rows(float* __restrict__ x, long HW, int D, int negate) {
+  const int lane = threadIdx.x & 63;
+  const long p = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= HW) return;
+  float* v = x + p * D;
+  float m = -__builtin_huge_valf();
+  for (int k = lane; k < D; k += 64) m = fmaxf(m, v[k]);
+  m = wave_max(m);
+  float s = 0.f;
+  for (int k = lane; k < D; k += 64) s += __expf(v[k] - m);
+  s = wave_sum(s);
+  const float inv = (negate ? -1.f : 1.f) / s;
+  for (int k = lane; k < D; k += 64) v[k] = __expf(v[k] - m) * inv;
+}
+
+// RGBA float4 image (row pitch in floats) -> planar C x H x W, first C channels.
+__global__ void k_c4_to_planar(const float* __restrict__ c4, int H, int W, long pitch, int C,
+                               float* __restrict__ out) {
+  const long n = (long)H * W;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long y = i / W, x = i - y * W;
+    const float4 v = *reinterpret_cast<const float4*>(c4 + y * pitch + 4 * x);
+    const float e[4] = {v.x, v.y, v.z, v.w};
+    for (int c = 0; c < C; ++c) out[c * n + i] = e[c];
+  }
+}
+
+// Channel-last (stride cs, fp32 or bf16) -> planar fp32 C x H x W.
+template <typename T>
+__global__ void k_nhwc_to_nchw(const T* __restrict__ in, long HW, int C, int cs, float* __restrict__ out) {
+  const long n = HW * C;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long c = i / HW, p = i - c * HW;
+    out[i] = (float)in[p * cs + c];
+  }
+}
+
+// Planar fp32 C x H x W -> channel-last fp32 with stride cs (pad channels zeroed).
+__global__ void k_nchw_to_nhwc(const float* __restrict__ in, long HW, int C, int cs, float* __restrict__ out) {
+  const long n = HW * cs;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long p = i / cs;
+    const int c = (int)(i - p * cs);
+    out[i] = c < C ? in[(long)c * HW + p] : 0.f;
+  }
+}
+
+}  // namespace ccs
+
+using namespace ccs;
+
+extern "C" int ccs_bias_forward(ccs_handle_t h, float* y, int layout, int C, int H, int W, int cs,
+                                const float* bias) {
+  CCS_CHECK(h && y && bias, CCS_ERR_ARG, "ccs_bias_forward: NULL argument");
+  CCS_CHECK(C >= 1 && H >= 1 && W >= 1, CCS_ERR_ARG, "ccs_bias_forward: bad shape");
+  const long HW = (long)H * W;
+  if (layout == CCS_LAYOUT_NHWC) {
+    CCS_CHECK(cs >= C, CCS_ERR_ARG, "ccs_bias_forward: channel stride %d < C=%d", cs, C);
+    k_bias_nhwc<<<grid_for(HW * C), kThreads, 0, h->stream>>>(y, HW, C, cs, bias);
+  } else {
+    CCS_CHECK(layout == CCS_LAYOUT_NCHW, CCS_ERR_ARG, "ccs_bias_forward: unknown layout %d", layout);
+    k_bias_nchw<<<grid_for(HW * C), kThreads, 0, h->stream>>>(y, HW, C, bias);
+  }
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_tanh_forward(ccs_handle_t h, float* y, long count) {
+  CCS_CHECK(h && y && count >= 0, CCS_ERR_ARG, "ccs_tanh_forward: bad argument");
+  if (count == 0) return CCS_OK;
+  k_tanh<<<grid_for(count), kThreads, 0, h->stream>>>(y, count);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_softmax_forward(ccs_handle_t h, float* x, long HW, int D, int negate) {
+  CCS_CHECK(h && x && HW >= 0 && D >= 1, CCS_ERR_ARG, "ccs_softmax_forward: bad argument");
+  if (HW == 0) return CCS_OK;
+  k_softmax_rows<<<(unsigned)((HW + 3) / 4), 256, 0, h->stream>>>(x, HW, D, negate);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_c4_to_planar(ccs_handle_t h, const float* c4, int H, int W, long pitch, int C,
+                                float* out) {
+  CCS_CHECK(h && c4 && out, CCS_ERR_ARG, "ccs_c4_to_planar: NULL argument");
+  CCS_CHECK(H >= 1 && W >= 1 && C >= 1 && C <= 4 && pitch >= 4L * W && pitch % 4 == 0, CCS_ERR_ARG,
+            "ccs_c4_to_planar: bad shape (H=%d W=%d C=%d pitch=%ld)", H, W, C, pitch);
+  CCS_CHECK(((uintptr_t)c4 & 15) == 0, CCS_ERR_ARG, "ccs_c4_to_planar: image must be 16-byte aligned");
+  k_c4_to_planar<<<grid_for((long)H * W), kThreads, 0, h->stream>>>(c4, H, W, pitch, C, out);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_nhwc_to_nchw(ccs_handle_t h, const void* in, int precision, int C, int cs, int H, int W,
+                                float* out) {
+  CCS_CHECK(h && in && out, CCS_ERR_ARG, "ccs_nhwc_to_nchw: NULL argument");
+  CCS_CHECK(C >= 1 && cs >= C && H >= 1 && W >= 1, CCS_ERR_ARG, "ccs_nhwc_to_nchw: bad shape");
+  const long HW = (long)H * W;
+  if (precision == CCS_BF16)
+    k_nhwc_to_nchw<bf16><<<grid_for(HW * C), kThreads, 0, h->stream>>>((const bf16*)in, HW, C, cs, out);
+  else if (precision == CCS_FP32)
+    k_nhwc_to_nchw<float><<<grid_for(HW * C), kThreads, 0, h->stream>>>((const float*)in, HW, C, cs, out);
+  else
+    return fail(CCS_ERR_ARG, "ccs_nhwc_to_nchw: unknown precision %d", precision);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_nchw_to_nhwc(ccs_handle_t h, const float* in, int C, int cs, int H, int W, float* out) {
+  CCS_CHECK(h && in && out, CCS_ERR_ARG, "ccs_nchw_to_nhwc: NULL argument");
+  CCS_CHECK(C >= 1 && cs >= C && H >= 1 && W >= 1, CCS_ERR_ARG, "ccs_nchw_to_nhwc: bad shape");
+  const long HW = (long)H * W;
+  k_nchw_to_nhwc<<<grid_for(HW * cs), kThreads, 0, h->stream>>>(in, HW, C, cs, out);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}

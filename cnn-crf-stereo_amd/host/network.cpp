@@ -1,0 +1,359 @@
+// StereoNet / ColorStereoNet (stereo/stereonet.cpp, stereo/colorstereonet.cpp)
+// and the Stereo facade (stereo/stereo.cpp) over the operator classes.
+//
+// The network keeps the reference's operator lists (Convolution, Bias,
+// Activation per layer; StereoCorrelation, Softmax after the towers) so that
+// setAllParams walks the same arr_<i> keys, but it runs them fused: each
+// Convolution applies its layer's bias and tanh in the MFMA epilogue, and the
+// correlation applies the -softmax over D in its epilogue.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ccs_stereo.hpp"
+
+namespace ccs {
+namespace {
+
+[[noreturn]] void arg_fail(const std::string& msg) { throw Error(CCS_ERR_ARG, msg); }
+
+int log2i(int num) {  // floor(log2(num)) for num >= 1, 0 for num <= 1 (stereo/stereo.cpp:56-62)
+  int r = 0;
+  while (num >>= 1) ++r;
+  return r;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- StereoNet
+
+StereoNet::StereoNet(int numLayers, int in, int ic, int ih, int iw)
+    : m_handle(new Handle(nullptr)), m_numLayers(numLayers), m_in(in), m_ic(ic), m_ih(ih), m_iw(iw) {
+  if (numLayers != 3 && numLayers != 5 && numLayers != 7)
+    arg_fail("StereoNet: numLayers must be 3, 5 or 7 (stereo/stereo.cpp:198-202)");
+  if (in != 1) throw Error(CCS_ERR_UNSUPPORTED, "StereoNet: batch size 1 only");
+  if (ic < 1 || ic > 4) throw Error(CCS_ERR_UNSUPPORTED, "StereoNet: 1 to 4 input channels");
+}
+
+StereoNet::~StereoNet() {
+  for (auto* v : {&m_leftOps, &m_rightOps, &m_lrOps})
+    for (Operator* op : *v) delete op;
+}
+
+void StereoNet::initNet(float min_disp, float max_disp, float step, int rect_corr, Tensor* d_unaryOut,
+                        Tensor* d_pairwiseOut) {
+  if (m_initialized) arg_fail("StereoNet::initNet: already initialised");
+  if (d_pairwiseOut)
+    throw Error(CCS_ERR_UNSUPPORTED, "StereoNet::initNet: the pairwise CNN (COLORPAIR models) is not implemented");
+  for (std::vector<Operator*>* ops : {&m_leftOps, &m_rightOps}) {
+    // layer 1: 3x3, ic -> 100; layers 2..numLayers: 2x2, 100 -> 100 (stereo/stereonet.cpp:65-132)
+    for (int l = 0; l < m_numLayers; ++l) {
+      Convolution* conv = l == 0 ? new Convolution(Convolution::CONVOLUTION, m_in, m_ic, m_ih, m_iw, 100, m_ic, 3, 3)
+                                 : new Convolution(Convolution::CONVOLUTION, ops->back()->outTensorDesc(), 100, 100, 2, 2);
+      conv->setPrecision(m_precision);
+      ops->push_back(conv);
+      Bias* bias = new Bias(conv->outTensorDesc(), 1, 100, 1, 1);
+      ops->push_back(bias);
+      ops->push_back(new Activation(conv->outTensorDesc(), Activation::TANH));
+      conv->setFusedEpilogue(bias->getParams(), true);
+    }
+  }
+  std::vector<TensorDesc> corrInputs = {m_leftOps.back()->outTensorDesc(), m_rightOps.back()->outTensorDesc()};
+  auto* corr = new StereoCorrelation(corrInputs, d_unaryOut, min_disp, max_disp, step, rect_corr);
+  corr->setNegSoftmaxEpilogue(true);
+  m_lrOps.push_back(corr);
+  auto* sm = new Softmax(corr->outTensorDesc(), Softmax::CHANNEL);
+  sm->setNegate(true);
+  m_lrOps.push_back(sm);
+  m_initialized = true;
+}
+
+void StereoNet::setPrecision(Precision p) {
+  m_precision = p;
+  for (auto* v : {&m_leftOps, &m_rightOps})
+    for (Operator* op : *v)
+      if (op->type() == Operator::CONVOLUTION) static_cast<Convolution*>(op)->setPrecision(p);
+}
+
+void StereoNet::setAllParams(const std::string& npzPath) {
+  if (!m_initialized) arg_fail("StereoNet::setAllParams: call initNet first");
+  const std::map<std::string, HostArray> all = npz::load(npzPath);
+  int param_idx = 0;
+  for (size_t i = 0; i < m_leftOps.size(); ++i) {
+    if (m_leftOps[i]->type() == Operator::ACTIVATION) continue;
+    const std::string key = "arr_" + std::to_string(param_idx);
+    auto it = all.find(key);
+    if (it == all.end()) throw Error(CCS_ERR_IO, npzPath + ": no array '" + key + "'");
+    if (m_verbose) std::printf("Unary: params_idx %d\n", param_idx);
+    // The same array goes into both towers (shared weights, stereo/stereonet.cpp:196-206).
+    m_leftOps[i]->setParams(it->second);
+    m_rightOps[i]->setParams(it->second);
+    ++param_idx;
+  }
+  // Trailing arrays (the learned L1, L2 of CRF models) are not network parameters.
+}
+
+void StereoNet::setDisparities(float min_disp, float max_disp, float step, Tensor* d_out) {
+  if (m_lrOps.empty() || m_lrOps[0]->type() != Operator::STEREO_CORRELATION)
+    arg_fail("StereoNet::setDisparities: no StereoCorrelation operation (call initNet first)");
+  static_cast<StereoCorrelation*>(m_lrOps[0])->setDisparities(min_disp, max_disp, step, d_out);
+}
+
+Tensor* StereoNet::performPrediction(Tensor* d_inputLeft, Tensor* d_inputRight) {
+  if (!m_initialized) arg_fail("StereoNet: call initNet first");
+  Handle& h = *m_handle;
+  Tensor* feats[2] = {d_inputLeft, d_inputRight};
+  for (int t = 0; t < 2; ++t) {
+    std::vector<Operator*>& ops = t == 0 ? m_leftOps : m_rightOps;
+    Tensor* x = feats[t];
+    Convolution* prev = nullptr;
+    for (Operator* op : ops) {
+      if (op->type() != Operator::CONVOLUTION) continue;  // bias + tanh run in the convolution epilogue
+      auto* conv = static_cast<Convolution*>(op);
+      x = conv->forward(x, h);
+      if (prev && m_allowGc) prev->deleteOutputMemory();  // freed once its consumer is queued
+      prev = conv;
+    }
+    feats[t] = x;
+  }
+  std::vector<Tensor*> in{feats[0], feats[1]};
+  Tensor* vol = m_lrOps[0]->forward(in, h);  // correlation + fused -softmax
+  if (m_allowGc) {
+    h.synchronize();
+    for (auto* v : {&m_leftOps, &m_rightOps})
+      if (!v->empty())
+        for (Operator* op : *v)
+          if (op->type() == Operator::CONVOLUTION) static_cast<Convolution*>(op)->deleteOutputMemory();
+  }
+  return vol;
+}
+
+Tensor* StereoNet::predictPrepared(Tensor* d_left, Tensor* d_right) { return performPrediction(d_left, d_right); }
+
+Tensor* StereoNet::predictPlanar(const float* d_left, const float* d_right) {
+  Handle& h = *m_handle;
+  TensorDesc d(1, m_ic, m_ih, m_iw);
+  Tensor l(d), r(d);
+  check(ccs_pad_normalize(h.get(), d_left, m_ic, m_ih, m_iw, 0, l.data(), nullptr), "StereoNet: normalise");
+  check(ccs_pad_normalize(h.get(), d_right, m_ic, m_ih, m_iw, 0, r.data(), nullptr), "StereoNet: normalise");
+  Tensor* out = performPrediction(&l, &r);
+  h.synchronize();  // l, r are freed on return
+  return out;
+}
+
+// ----------------------------------------------------------- ColorStereoNet
+
+ColorStereoNet::ColorStereoNet(int numLayers, int in, int ic, int ih, int iw) : StereoNet(numLayers, in, ic, ih, iw) {
+  if (ic == 5) throw Error(CCS_ERR_UNSUPPORTED, "ColorStereoNet: the x/y-input (5-channel) model is not implemented");
+  if (ic != 3) arg_fail("ColorStereoNet: ic must be 3");
+}
+
+Tensor* ColorStereoNet::predict(const float* d_left_c4, const float* d_right_c4, long pitch) {
+  Handle& h = *m_handle;
+  if (pitch == 0) pitch = 4L * m_iw;
+  TensorDesc d(1, 3, m_ih, m_iw);
+  Tensor l(d), r(d);
+  // C4 -> planar RGB, then per-channel zero mean / unit std (stereo/colorstereonet.cpp:53-68).
+  check(ccs_c4_to_planar(h.get(), d_left_c4, m_ih, m_iw, pitch, 3, l.data()), "ColorStereoNet: unpack");
+  check(ccs_c4_to_planar(h.get(), d_right_c4, m_ih, m_iw, pitch, 3, r.data()), "ColorStereoNet: unpack");
+  check(ccs_pad_normalize(h.get(), l.data(), 3, m_ih, m_iw, 0, l.data(), nullptr), "ColorStereoNet: normalise");
+  check(ccs_pad_normalize(h.get(), r.data(), 3, m_ih, m_iw, 0, r.data(), nullptr), "ColorStereoNet: normalise");
+  Tensor* out = performPrediction(&l, &r);
+  h.synchronize();
+  return out;
+}
+
+Tensor* ColorStereoNet::predictHost(const float* left_c4, const float* right_c4) {
+  TensorDesc d(1, 4, m_ih, m_iw, Layout::NHWC);
+  Tensor l(d), r(d);
+  l.copyFromHost(left_c4);
+  r.copyFromHost(right_c4);
+  return predict(l.data(), r.data());
+}
+
+// ------------------------------------------------------------------- Stereo
+
+int label_count(float disp_min, float disp_step, float disp_max, float* disp_max_out) {
+  if (disp_step <= 0 || disp_max < disp_min) arg_fail("label_count: need step > 0 and max >= min");
+  int depth = (int)((disp_max - disp_min) / disp_step) + 1;
+  depth = std::min(256, 1 << (log2i(depth - 1) + 1));
+  if (disp_max_out) *disp_max_out = disp_min + (depth - 1) * disp_step;
+  return depth;
+}
+
+Stereo::Stereo() = default;
+
+Stereo::~Stereo() {
+  delete stereoNet_;
+  for (Tensor* t : {img1_, img2_, img1_color_, img2_color_, costvolume_, pairwise_, wx_, wy_, out_, labels_, pad1_, pad2_})
+    delete t;
+}
+
+void Stereo::initialize(int width, int height, float disp_min, float disp_step, float disp_max) {
+  if (width < 1 || height < 1 || disp_step <= 0 || disp_max < disp_min) arg_fail("Stereo::initialize: bad arguments");
+  if (!handle_) handle_.reset(new Handle(nullptr));
+  width_ = width;
+  height_ = height;
+  depth_ = label_count(disp_min, disp_step, disp_max, &disp_max_);
+  disp_min_ = disp_min;
+  disp_step_ = disp_step;
+  img1_ = new Tensor(TensorDesc(1, 1, height, width));
+  img2_ = new Tensor(TensorDesc(1, 1, height, width));
+  img1_color_ = new Tensor(TensorDesc(1, 4, height, width, Layout::NHWC));
+  img2_color_ = new Tensor(TensorDesc(1, 4, height, width, Layout::NHWC));
+  costvolume_ = new Tensor(TensorDesc(1, depth_, height, width, Layout::NHWC));
+  // wx and wy are the two planes of one 1 x 2 x H x W buffer (stereo/stereo.cpp:84-86).
+  pairwise_ = new Tensor(TensorDesc(1, 2, height, width));
+  wx_ = new Tensor(pairwise_->data(), TensorDesc(1, 1, height, width));
+  wy_ = new Tensor(pairwise_->data() + (size_t)width * height, TensorDesc(1, 1, height, width));
+  out_ = new Tensor(TensorDesc(1, 1, height, width));
+  labels_ = new Tensor(TensorDesc(1, 1, height, width));  // int32 storage
+}
+
+void Stereo::setImages(const float* img1, const float* img2) {
+  if (!img1_) arg_fail("Stereo: call initialize first");
+  img1_->copyFromHost(img1);
+  img2_->copyFromHost(img2);
+  have_color_ = false;
+}
+
+void Stereo::setColorImages(const float* img1_c4, const float* img2_c4) {
+  if (!img1_) arg_fail("Stereo: call initialize first");
+  img1_color_->copyFromHost(img1_c4);
+  img2_color_->copyFromHost(img2_c4);
+  // Gray images for the edge weights (stereo/stereo.cpp:120-128).
+  Tensor planar(TensorDesc(1, 3, height_, width_));
+  for (int i = 0; i < 2; ++i) {
+    check(ccs_c4_to_planar(handle_->get(), (i ? img2_color_ : img1_color_)->data(), height_, width_, 4L * width_, 3,
+                           planar.data()),
+          "Stereo: unpack");
+    check(ccs_rgb_to_gray(handle_->get(), planar.data(), height_, width_, (i ? img2_ : img1_)->data()),
+          "Stereo: rgb2gray");
+  }
+  handle_->synchronize();
+  have_color_ = true;
+}
+
+void Stereo::setWithXy(bool value) {
+  if (value) throw Error(CCS_ERR_UNSUPPORTED, "Stereo: the x/y-input model is not implemented");
+}
+
+int Stereo::padval() const {
+  switch (num_cnn_layers_) {  // stereo/stereo.cpp:204-217
+    case 3: return 2;
+    case 5: return 3;
+    case 7: return 4;
+  }
+  arg_fail("Stereo: numLayers must be 3, 5 or 7 (stereo/stereo.cpp:198-202)");
+}
+
+void Stereo::ensureNet() {
+  if (stereoNet_) return;
+  const int p = padval();
+  const int ic = matching_function_ == COLORCNN ? 3 : 1;
+  stereoNet_ = ic == 3 ? new ColorStereoNet(num_cnn_layers_, 1, 3, height_ + 2 * p, width_ + 2 * p)
+                       : new StereoNet(num_cnn_layers_, 1, 1, height_ + 2 * p, width_ + 2 * p);
+  stereoNet_->setPrecision(precision_);
+  stereoNet_->setVerbose(verbose_);
+  stereoNet_->initNet(disp_min_, disp_max_, disp_step_, 0, costvolume_);
+  std::string params = cnn_parameter_file_base_name_;
+  if (params.size() < 4 || params.compare(params.size() - 4, 4, ".npz") != 0) params += ".npz";
+  stereoNet_->setAllParams(params);
+  stereoNet_->setAllowGc(allowCnnGc_);
+}
+
+void Stereo::computeVolume(bool lr) {
+  if (!img1_) arg_fail("Stereo: call initialize first");
+  if (matching_function_ == LOAD) return;
+  if (matching_function_ != COLORCNN && matching_function_ != GRAYCNN)
+    throw Error(CCS_ERR_UNSUPPORTED, "Stereo::computeVolume: matching function not implemented");
+  if (matching_function_ == COLORCNN && !have_color_) arg_fail("Stereo: COLORCNN needs setColorImages");
+  ensureNet();
+  Handle& h = stereoNet_->handle();
+  const int p = padval();
+  const int ic = matching_function_ == COLORCNN ? 3 : 1;
+  const int Hp = height_ + 2 * p, Wp = width_ + 2 * p;
+  if (lr)
+    stereoNet_->setDisparities(disp_min_, disp_max_, disp_step_, costvolume_);
+  else
+    stereoNet_->setDisparities(-disp_max_, -disp_min_, disp_step_, costvolume_);
+  // Reference image first: left for lr, right for rl (stereo/stereo.cpp:260-318).
+  Tensor* src[2];
+  std::unique_ptr<Tensor> planar[2];
+  for (int i = 0; i < 2; ++i) {
+    const bool first_is_left = (i == 0) == lr;
+    if (ic == 3) {
+      planar[i].reset(new Tensor(TensorDesc(1, 3, height_, width_)));
+      check(ccs_c4_to_planar(h.get(), (first_is_left ? img1_color_ : img2_color_)->data(), height_, width_,
+                             4L * width_, 3, planar[i]->data()),
+            "Stereo: unpack");
+      src[i] = planar[i].get();
+    } else {
+      src[i] = first_is_left ? img1_ : img2_;
+    }
+  }
+  if (!pad1_) {
+    pad1_ = new Tensor(TensorDesc(1, ic, Hp, Wp));
+    pad2_ = new Tensor(TensorDesc(1, ic, Hp, Wp));
+  }
+  // Mirror pad + per-channel normalisation in one pass each.
+  check(ccs_pad_normalize(h.get(), src[0]->data(), ic, height_, width_, p, pad1_->data(), nullptr), "Stereo: pad");
+  check(ccs_pad_normalize(h.get(), src[1]->data(), ic, height_, width_, p, pad2_->data(), nullptr), "Stereo: pad");
+  stereoNet_->predictPrepared(pad1_, pad2_);
+  h.synchronize();
+}
+
+void Stereo::computeWeights(float alpha, float beta, float lambda, bool lr) {
+  Handle& h = *handle_;
+  check(ccs_edge_weights(h.get(), (lr ? img1_ : img2_)->data(), height_, width_, 0.7f, alpha, beta, lambda,
+                         wx_->data(), wy_->data()),
+        "Stereo::computeWeights");
+}
+
+void Stereo::computeSlackPropagation(float C1, float C2, float delta, int iterations, float alpha, float beta,
+                                     float lambda_reg, bool lr) {
+  computeVolume(lr);
+  computeWeights(alpha, beta, lambda_reg, lr);
+  Handle& h = *handle_;
+  Tensor lb(TensorDesc(1, 1, 1, 2));
+  check(ccs_slackprop(h.get(), costvolume_->data(), wx_->data(), wy_->data(), height_, width_, depth_, C1, C2,
+                      (int)std::lround(delta), iterations, (int*)labels_->raw(), nullptr, nullptr, (double*)lb.raw()),
+        "Stereo::computeSlackPropagation");
+  // remapSolution (stereo/stereo.cpp:155-158): lr -> min + l step; rl -> -max + l step.
+  check(ccs_remap(h.get(), (const int*)labels_->raw(), height_, width_, lr ? disp_min_ : -disp_max_, disp_step_,
+                  out_->data()),
+        "Stereo: remap");
+  h.synchronize();
+  if (hipMemcpy(&lower_bound_, lb.raw(), sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    throw Error(CCS_ERR_HIP, "Stereo: reading the bound");
+  if (verbose_) std::printf("CRF lower bound: %.6f\n", lower_bound_);
+}
+
+void Stereo::computeArgMax(bool lr) {
+  computeVolume(lr);
+  check(ccs_argmin(handle_->get(), costvolume_->data(), height_, width_, depth_, lr ? disp_min_ : -disp_max_,
+                   disp_step_, out_->data()),
+        "Stereo::computeArgMax");
+  handle_->synchronize();
+}
+
+bool Stereo::loadVolumeFromFile(const std::string& filename) {
+  // LOAD matching: a .npy cost volume of H x W x D float32/float64 values.
+  if (!costvolume_) arg_fail("Stereo: call initialize first");
+  HostArray a;
+  try {
+    a = npz::load_npy(filename);
+  } catch (const Error&) {
+    return false;
+  }
+  if (a.size() != (size_t)height_ * width_ * depth_) return false;
+  costvolume_->copyFromHost(a.data.data());
+  return true;
+}
+
+}  // namespace ccs

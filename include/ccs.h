@@ -139,7 +139,7 @@ int ccs_correlation_volume_backward(ccs_handle_t h, const float* f0, const float
                                     int W, int C, int disp_min, int disp_step, int D,
                                     const float* G, float* df0, float* df1);
 
-/* Cross entropy of softmax_D(S) at target labels (target < 0 ignored):
+/* Cross entropy of softmax_D(S) at target labels (targets outside [0, D) ignored):
  * loss (device, 1 double) and dS = softmax - onehot (H x W x D). */
 int ccs_softmax_xent(ccs_handle_t h, const float* S, int HW, int D, const int* target,
                      double* loss, float* dS);
@@ -184,6 +184,14 @@ int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, const float
                         int H, int W, int D, float L1, float L2, int delta, int direction,
                         int* labels, double* chain_sum, double* kept_sum);
 
+/* Row decode of caller-held unaries (as left by ccs_slackprop_sweep) with the
+ * min-marginal gap: labels (H x W int32) = lowest argmin of the row min-marginals,
+ * gap (H x W float32) = second-best minus best min-marginal, >= 0.  The gap marks
+ * pixels whose label is decided by a near tie (parity checks use it); the decode
+ * step of slack_prop_2D_alg (stereo/stereo.cpp:153-154) with this extra output. */
+int ccs_slackprop_decode_gap(ccs_handle_t h, const float* unaries, const float* wx, int H, int W,
+                             int D, float L1, float L2, int delta, int* labels, float* gap);
+
 /* CRF energy of a labelling, accumulated in float64 (device double). */
 int ccs_crf_energy(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H,
                    int W, int D, const int* labels, float L1, float L2, int delta,
@@ -195,6 +203,37 @@ int ccs_crf_energy(ccs_handle_t h, const float* cost, const float* wx, const flo
 int ccs_crf_ssvm_grad(ccs_handle_t h, const int* xhat, const int* xstar, const float* wx,
                       const float* wy, int H, int W, int D, float L1, float L2, int delta,
                       float* dC, double* dL, float* dwx, float* dwy);
+
+/* ---- stand-alone operator forms and layout moves ------------------------ */
+
+/* y += bias per channel, in place.  layout CCS_LAYOUT_NCHW (planar C x H x W) or
+ * CCS_LAYOUT_NHWC (channel-last with channel stride cs >= C).  Replaces
+ * Bias::forward (stereo/operators/bias.cpp:58-66); the feature towers fuse it
+ * into ccs_conv2d_forward instead. */
+int ccs_bias_forward(ccs_handle_t h, float* y, int layout, int C, int H, int W, int cs,
+                     const float* bias);
+
+/* y = tanh(y) over count floats, in place.  Replaces Activation::forward with
+ * CUDNN_ACTIVATION_TANH (stereo/operators/activation.cpp:25,33-40). */
+int ccs_tanh_forward(ccs_handle_t h, float* y, long count);
+
+/* Softmax over the D contiguous channels of each of HW pixels, in place
+ * (max-subtracted); negate != 0 also applies cost = -p.  Replaces
+ * Softmax::forward (stereo/operators/softmax.cpp:30-38, CHANNEL mode, NHWC)
+ * followed by cuda::negateTensor (stereo/utils.cu:289-292). */
+int ccs_softmax_forward(ccs_handle_t h, float* x, long HW, int D, int negate);
+
+/* RGBA float image (16-byte aligned, row pitch in floats) -> planar C x H x W
+ * (C <= 4 leading channels).  The copy at the top of ColorStereoNet::predict
+ * (stereo/colorstereonet.cpp:53-54). */
+int ccs_c4_to_planar(ccs_handle_t h, const float* c4, int H, int W, long pitch, int C, float* out);
+
+/* Channel-last feature map (stride cs, CCS_FP32 or CCS_BF16) -> planar fp32
+ * C x H x W, and planar fp32 -> channel-last fp32 (pad channels zeroed).  Layout
+ * moves for callers that hold NCHW tensors (iu::TensorGpu_32f NCHW). */
+int ccs_nhwc_to_nchw(ccs_handle_t h, const void* in, int precision, int C, int cs, int H, int W,
+                     float* out);
+int ccs_nchw_to_nhwc(ccs_handle_t h, const float* in, int C, int cs, int H, int W, float* out);
 
 /* ---- decode -------------------------------------------------------------- */
 

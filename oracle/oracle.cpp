@@ -228,10 +228,11 @@ extern "C" double or_softmax_xent(const double* S, long HW, int D, const int* ta
     for (int k = 0; k < D; ++k) s += std::exp(v[k] - m);
     const double lse = m + std::log(s);
     const int t = target[p];
-    if (t >= 0) loss += lse - v[t];
+    const bool valid = t >= 0 && t < D;  // other targets mark pixels without ground truth
+    if (valid) loss += lse - v[t];
     if (dS)
       for (int k = 0; k < D; ++k)
-        dS[p * D + k] = t >= 0 ? std::exp(v[k] - lse) - (k == t ? 1.0 : 0.0) : 0.0;
+        dS[p * D + k] = valid ? std::exp(v[k] - lse) - (k == t ? 1.0 : 0.0) : 0.0;
   }
   return loss;
 }

@@ -60,7 +60,7 @@ void or_correlation_bwd(const double* f0, const double* f1, int C, int H, int W,
 /* cost = -softmax_D(vol), in place: stereo/operators/softmax.cpp:30-38, stereo/utils.cu:289-292. */
 void or_neg_softmax(double* vol, long HW, int D);
 
-/* Cross entropy of softmax_D(S) at target labels (target < 0 ignored).
+/* Cross entropy of softmax_D(S) at target labels (targets outside [0, D) ignored).
  * Returns the summed loss; dS (H*W*D) receives dLoss/dS. */
 double or_softmax_xent(const double* S, long HW, int D, const int* target, double* dS);
 

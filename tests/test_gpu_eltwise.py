@@ -1,0 +1,68 @@
+"""Stand-alone operator forms and layout moves (bias, tanh, softmax, C4 and
+NHWC/NCHW conversions) through the C ABI against torch float32 references of
+the same op.  Tolerances: exact for moves and bias adds, 2e-6 relative for
+tanh and softmax (float32 exp)."""
+import numpy as np
+import pytest
+import torch
+
+from cnn_crf_stereo_amd.capi import BF16, FP32, LAYOUT_NCHW, LAYOUT_NHWC, call, ptr
+
+pytestmark = pytest.mark.gpu
+
+
+def test_bias_both_layouts(handle):
+    g = torch.Generator(device="cuda").manual_seed(1)
+    C, H, W, cs = 5, 7, 9, 8
+    b = torch.randn(C, device="cuda", generator=g)
+    y = torch.randn(C, H, W, device="cuda", generator=g)
+    want = y + b[:, None, None]
+    call("ccs_bias_forward", handle.h, ptr(y), LAYOUT_NCHW, C, H, W, 0, ptr(b))
+    torch.testing.assert_close(y, want, rtol=0, atol=0)
+    z = torch.randn(H, W, cs, device="cuda", generator=g)
+    want = z.clone()
+    want[:, :, :C] += b
+    call("ccs_bias_forward", handle.h, ptr(z), LAYOUT_NHWC, C, H, W, cs, ptr(b))
+    torch.testing.assert_close(z, want, rtol=0, atol=0)  # pad channels untouched
+
+
+def test_tanh(handle):
+    x = torch.linspace(-12, 12, 10007, device="cuda")
+    want = torch.tanh(x)
+    call("ccs_tanh_forward", handle.h, ptr(x), x.numel())
+    torch.testing.assert_close(x, want, rtol=2e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("D,negate", [(128, 1), (37, 0), (256, 1), (1, 1)])
+def test_softmax_channels(handle, D, negate):
+    g = torch.Generator(device="cuda").manual_seed(D)
+    x = torch.randn(13, 17, D, device="cuda", generator=g) * 6
+    want = torch.softmax(x, -1) * (-1 if negate else 1)
+    call("ccs_softmax_forward", handle.h, ptr(x), 13 * 17, D, negate)
+    torch.testing.assert_close(x, want, rtol=2e-6, atol=1e-7)
+
+
+def test_layout_moves(handle):
+    g = torch.Generator(device="cuda").manual_seed(3)
+    C, cs, H, W = 100, 104, 6, 11
+    f = torch.randn(H, W, cs, device="cuda", generator=g)
+    out = torch.empty(C, H, W, device="cuda")
+    call("ccs_nhwc_to_nchw", handle.h, ptr(f), FP32, C, cs, H, W, ptr(out))
+    torch.testing.assert_close(out, f[:, :, :C].permute(2, 0, 1), rtol=0, atol=0)
+    fb = f.to(torch.bfloat16)
+    call("ccs_nhwc_to_nchw", handle.h, ptr(fb), BF16, C, cs, H, W, ptr(out))
+    torch.testing.assert_close(out, fb[:, :, :C].float().permute(2, 0, 1), rtol=0, atol=0)
+    back = torch.full((H, W, cs), 7.0, device="cuda")
+    call("ccs_nchw_to_nhwc", handle.h, ptr(out), C, cs, H, W, ptr(back))
+    torch.testing.assert_close(back[:, :, :C], fb[:, :, :C].float(), rtol=0, atol=0)
+    assert torch.all(back[:, :, C:] == 0)
+
+
+def test_c4_to_planar(handle):
+    g = torch.Generator(device="cuda").manual_seed(4)
+    H, W, pitch = 5, 7, 4 * 8  # padded rows
+    img = torch.randn(H, pitch, device="cuda", generator=g)
+    out = torch.empty(3, H, W, device="cuda")
+    call("ccs_c4_to_planar", handle.h, ptr(img), H, W, pitch, 3, ptr(out))
+    want = img[:, :4 * W].reshape(H, W, 4)[:, :, :3].permute(2, 0, 1)
+    torch.testing.assert_close(out, want, rtol=0, atol=0)
