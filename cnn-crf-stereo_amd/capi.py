@@ -45,10 +45,12 @@ SIGNATURES = {
     "ccs_slackprop_decode_gap": (_I, [_P, _P, _P, _I, _I, _I, _F, _F, _I, _P, _P]),
     "ccs_slackprop_sweep": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _I, _P, _P, _P]),
     "ccs_crf_energy": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _F, _F, _I, _P]),
+    "ccs_crf_loss_augment": (_I, [_P, _P, _P, ctypes.c_long, _I, _F, _P]),
     "ccs_crf_ssvm_grad": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _P, _P, _P, _P]),
     "ccs_bias_forward": (_I, [_P, _P, _I, _I, _I, _I, _I, _P]),
     "ccs_tanh_forward": (_I, [_P, _P, ctypes.c_long]),
     "ccs_softmax_forward": (_I, [_P, _P, ctypes.c_long, _I, _I]),
+    "ccs_neg_softmax_backward": (_I, [_P, _P, _P, ctypes.c_long, _I, _P]),
     "ccs_c4_to_planar": (_I, [_P, _P, _I, _I, ctypes.c_long, _I, _P]),
     "ccs_nhwc_to_nchw": (_I, [_P, _P, _I, _I, _I, _I, _I, _P]),
     "ccs_nchw_to_nhwc": (_I, [_P, _P, _I, _I, _I, _I, _P]),

@@ -56,6 +56,17 @@ __device__ __forceinline__ float rho(int d, float L1, float L2, int delta) {
 }
 
 // Per-block float64 partial energy.
+// Loss-augmented unaries of structured-SVM training: out = C - margin [l != t_p]
+// (Hamming loss); pixels without ground truth (t outside [0, D)) keep C.
+__global__ void k_loss_augment(const float* C, const int* __restrict__ t, long HW, int D, float margin, float* out) {
+  const long n = HW * D;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long p = i / D;
+    const int l = (int)(i - p * D), tp = t[p];
+    out[i] = C[i] - ((tp >= 0 && tp < D && l != tp) ? margin : 0.f);
+  }
+}
+
 __global__ void k_energy_partial(const float* __restrict__ C, const float* __restrict__ wx,
                                  const float* __restrict__ wy, int H, int W, int D, const int* __restrict__ lab,
                                  float L1, float L2, int delta, double* __restrict__ part) {
@@ -261,6 +272,17 @@ extern "C" int ccs_slackprop_forward(ccs_handle_t h, const float* cost_volume, c
   CCS_HIP(hipMemcpyAsync(&lbh, lb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   CCS_HIP(hipStreamSynchronize(h->stream));
   if (lower_bound) *lower_bound = (float)lbh;
+  return CCS_OK;
+}
+
+extern "C" int ccs_crf_loss_augment(ccs_handle_t h, const float* cost, const int* target, long HW, int D,
+                                    float margin, float* out) {
+  CCS_CHECK(h && cost && target && out && HW >= 0 && D >= 1, CCS_ERR_ARG, "ccs_crf_loss_augment: bad argument");
+  if (HW == 0) return CCS_OK;
+  const long n = HW * D;
+  const long nb = (n + 255) / 256;
+  k_loss_augment<<<(unsigned)(nb < 65536 ? nb : 65536), 256, 0, h->stream>>>(cost, target, HW, D, margin, out);
+  CCS_LAUNCH_CHECK();
   return CCS_OK;
 }
 

@@ -57,6 +57,22 @@ __global__ __launch_bounds__(256) void k_softmax_rows(float* __restrict__ x, lon
   for (int k = lane; k < D; k += 64) v[k] = __expf(v[k] - m) * inv;
 }
 
+// Adjoint of cost = -softmax(S) over D channels, one wave per pixel:
+// dS_j = p_j (sum_k p_k g_k - g_j) with p = -cost, g = dLoss/dcost.
+// g and dS may alias (each lane reads g[k] before it writes dS[k]).
+__global__ __launch_bounds__(256) void k_neg_softmax_bwd(const float* __restrict__ cost, const float* g, long HW,
+                                                         int D, float* dS) {
+  const int lane = threadIdx.x & 63;
+  const long p = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (p >= HW) return;
+  const float* c = cost + p * D;
+  const float* gp = g + p * D;
+  float s = 0.f;
+  for (int k = lane; k < D; k += 64) s -= c[k] * gp[k];
+  s = wave_sum(s);
+  for (int k = lane; k < D; k += 64) dS[p * D + k] = -c[k] * (s - gp[k]);
+}
+
 // RGBA float4 image (row pitch in floats) -> planar C x H x W, first C channels.
 __global__ void k_c4_to_planar(const float* __restrict__ c4, int H, int W, long pitch, int C,
                                float* __restrict__ out) {
@@ -121,6 +137,15 @@ extern "C" int ccs_softmax_forward(ccs_handle_t h, float* x, long HW, int D, int
   CCS_CHECK(h && x && HW >= 0 && D >= 1, CCS_ERR_ARG, "ccs_softmax_forward: bad argument");
   if (HW == 0) return CCS_OK;
   k_softmax_rows<<<(unsigned)((HW + 3) / 4), 256, 0, h->stream>>>(x, HW, D, negate);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_neg_softmax_backward(ccs_handle_t h, const float* cost, const float* dcost, long HW, int D,
+                                        float* dS) {
+  CCS_CHECK(h && cost && dcost && dS && HW >= 0 && D >= 1, CCS_ERR_ARG, "ccs_neg_softmax_backward: bad argument");
+  if (HW == 0) return CCS_OK;
+  k_neg_softmax_bwd<<<(unsigned)((HW + 3) / 4), 256, 0, h->stream>>>(cost, dcost, HW, D, dS);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }

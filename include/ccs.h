@@ -197,6 +197,13 @@ int ccs_crf_energy(ccs_handle_t h, const float* cost, const float* wx, const flo
                    int W, int D, const int* labels, float L1, float L2, int delta,
                    double* energy);
 
+/* Loss-augmented unaries for structured-SVM training: out = cost - margin [l != target]
+ * per pixel (Hamming loss; targets outside [0, D) leave the pixel unchanged);
+ * out may alias cost.  Running ccs_slackprop on out gives the most violating
+ * labelling xhat.  No reference counterpart (training lived outside the repo). */
+int ccs_crf_loss_augment(ccs_handle_t h, const float* cost, const int* target, long HW, int D, float margin,
+                         float* out);
+
 /* Structured-SVM subgradient of E(x*) - E(xhat) w.r.t. the unaries (dC, H x W x D,
  * overwritten), L1/L2 (dL, device, 2 doubles) and the edge weights (dwx, dwy,
  * H x W, optional).  No reference counterpart (training lived outside the repo). */
@@ -222,6 +229,12 @@ int ccs_tanh_forward(ccs_handle_t h, float* y, long count);
  * Softmax::forward (stereo/operators/softmax.cpp:30-38, CHANNEL mode, NHWC)
  * followed by cuda::negateTensor (stereo/utils.cu:289-292). */
 int ccs_softmax_forward(ccs_handle_t h, float* x, long HW, int D, int negate);
+
+/* Adjoint of cost = -softmax_D(S): dS = p (sum_k p_k g_k - g) per pixel with
+ * p = -cost and g = dcost (HW x D each; dS may alias dcost).  Backward of the
+ * fused epilogue of ccs_correlation_volume; no reference counterpart (the
+ * reference has no softmax backward). */
+int ccs_neg_softmax_backward(ccs_handle_t h, const float* cost, const float* dcost, long HW, int D, float* dS);
 
 /* RGBA float image (16-byte aligned, row pitch in floats) -> planar C x H x W
  * (C <= 4 leading channels).  The copy at the top of ColorStereoNet::predict
