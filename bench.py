@@ -139,7 +139,19 @@ def time_form(pkg, h, data, spec, params, precision, steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / steps
-    return ms, L.summary(), out
+    # Host images handed over per frame (pageable copies + the same frame).
+    hl, hr = data["left"], data["right"]
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(max(1, steps // 2)):
+        left.copy_(torch.from_numpy(hl))
+        right.copy_(torch.from_numpy(hr))
+        run_frame(ops, h, frame, left, right, Launches(False), A)
+    torch.cuda.synchronize()
+    ms_pcie = (time.perf_counter() - t2) * 1e3 / max(1, steps // 2)
+    summary = L.summary()
+    summary["_pcie_inclusive_ms"] = ms_pcie
+    return ms, summary, out
 
 
 def measured_traffic(workload):
@@ -194,8 +206,65 @@ def cpu_baseline(data, spec, params, pad, rows):
     r = O.slackprop(cost, wx, wy, 0.5, 2.0, 2, spec["iters"])
     dt = time.perf_counter() - t0
     Wd = left.shape[2]
-    return dict(seconds=dt, rows=rows, y0=y0, pd=rows * Wd * D, labels=r["labels"], lb=r["lb"],
-                left=left.astype(np.float32), right=right.astype(np.float32))
+    energy = O.energy(cost, wx, wy, r["labels"], 0.5, 2.0, 2)
+    return dict(seconds=dt, rows=rows, y0=y0, pd=rows * Wd * D, labels=r["labels"], lb=r["lb"], energy=energy,
+                feat_left=feats[0], cost=cost, left=left.astype(np.float32), right=right.astype(np.float32))
+
+
+def stage_errors(ops, h, fr, out, cb):
+    """Per-stage error of one GPU form against the FP64 oracle on the same band."""
+    import numpy as np
+    import torch
+
+    C = fr.C
+    fl = out["features"][0][:, :, :C].float().cpu().numpy().transpose(2, 0, 1).astype(np.float64)
+    ref_f = cb["feat_left"]
+    cost = out["cost"].cpu().numpy().astype(np.float64)
+    ref_c = cb["cost"]
+    e = ops.crf_energy(h, out["cost"], out["wx"], out["wy"], out["labels"], fr.L1, fr.L2, fr.delta)
+    torch.cuda.synchronize()
+    energy = float(e.item())
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    return {
+        "features_max_abs": float(np.abs(fl - ref_f).max()), "features_rel_l2": rel(fl, ref_f),
+        "cost_max_abs": float(np.abs(cost - ref_c).max()), "cost_rel_l2": rel(cost, ref_c),
+        "crf_energy_rel": abs(energy - cb["energy"]) / max(abs(cb["energy"]), 1e-300),
+        "crf_lb_rel": abs(float(out["lb"].item()) - cb["lb"]) / max(abs(cb["lb"]), 1e-300),
+    }
+
+
+def train_bench(pkg, h, workload, layers_n, arch, batch, steps, warmup, iters, seed=11):
+    """Training steps (forward, CRF hinge, backward through the hand-written
+    adjoints, SGD) on `batch` seeded frames of `workload`; FP32 master weights."""
+    import numpy as np
+    import torch
+    from cnn_crf_stereo_amd import autograd as AG
+    from cnn_crf_stereo_amd import synth
+
+    spec = dict(synth.WORKLOADS[workload])
+    D = spec["D"]
+    layers, _ = synth.architecture(layers_n, spec["C"], arch)
+    model = AG.StereoModel(synth.glorot_weights(layers, seed=seed))
+    frames = []
+    for i in range(batch):
+        d = synth.stereo_pair(spec, seed=2000 + i)
+        tgt = np.clip(np.rint(d["disp"]), 0, D - 1).astype(np.int32)
+        frames.append((torch.as_tensor(d["left"], device="cuda"), torch.as_tensor(d["right"], device="cuda"),
+                       torch.as_tensor(tgt, device="cuda")))
+    for _ in range(warmup):
+        AG.training_step(model, h, frames, D, lr=1e-4, iters=iters)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = [AG.training_step(model, h, frames, D, lr=1e-4, iters=iters) for _ in range(steps)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    free, total = torch.cuda.mem_get_info()
+    return {"workload": workload, "batch": batch, "layers": "%d x %s, 100 channels" % (layers_n, arch),
+            "crf_iterations": iters, "ms_per_step": round(ms, 2), "loss": [round(v, 5) for v in losses],
+            "device_memory_in_use_gb": round((total - free) / 1e9, 2), "dtype": "fp32"}
 
 
 def main():
@@ -210,6 +279,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="rows of the CPU-baseline band (0 = the whole frame)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-train", action="store_true", help="skip the training-step measurements")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -234,9 +304,10 @@ def main():
     pd_per_frame = float(H * W * D)
 
     ms_bf16, launches_bf16, out_bf16 = time_form(pkg, h, data, spec, params, "bf16", args.steps, args.warmup)
-    result_extra = {}
+    result_extra = {"pcie_inclusive_ms_per_frame": round(launches_bf16.pop("_pcie_inclusive_ms"), 4)}
     if not args.no_fp32:
         ms_fp32, launches_fp32, out_fp32 = time_form(pkg, h, data, spec, params, "fp32", args.steps, args.warmup)
+        launches_fp32.pop("_pcie_inclusive_ms")
         result_extra["fp32_ms_per_frame"] = round(ms_fp32, 4)
         result_extra["fp32_pixel_disp_per_s"] = pd_per_frame / (ms_fp32 / 1e3)
         result_extra["speedup_bf16_vs_fp32"] = round(ms_fp32 / ms_bf16, 3)
@@ -290,7 +361,7 @@ def main():
                           "%d CRF iterations: %.2f s" % (W, cb["rows"], cb["y0"], cb["y0"] + cb["rows"] - 1, D,
                                                           spec["iters"], cb["seconds"]))
         # The same band through both GPU forms, compared with the oracle's labels.
-        from cnn_crf_stereo_amd import pipeline
+        from cnn_crf_stereo_amd import ops, pipeline
 
         band_l = torch.as_tensor(cb["left"], device="cuda")
         band_r = torch.as_tensor(cb["right"], device="cuda")
@@ -302,7 +373,14 @@ def main():
             lab = o["labels"].cpu().numpy()
             band[prec] = bad_pixel_rate(lab, gt_band, D)
             band[prec + "_labels_equal_oracle"] = float((lab == cb["labels"]).mean())
+            result_extra.setdefault("errors_vs_fp64_oracle", {})[prec] = stage_errors(ops, h, fr, o, cb)
         result_extra["bad_px_band_vs_oracle"] = band
+
+    if not args.no_train:
+        result_extra["train_step"] = [
+            train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=3, warmup=1, iters=8),
+            train_bench(pkg, h, "rds-1488x992-d256", 7, "all3x3", batch=1, steps=2, warmup=1, iters=10),
+        ]
 
     line = {
         "metric": METRIC,

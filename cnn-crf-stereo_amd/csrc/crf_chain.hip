@@ -34,6 +34,11 @@
 namespace ccs {
 
 constexpr float kInfF = __builtin_huge_valf();
+// Relative tolerance under which two decoded min-marginals count as tied.
+#ifndef CCS_CRF_TIE_EPS
+#define CCS_CRF_TIE_EPS 1e-5f
+#endif
+constexpr float kTieEps = CCS_CRF_TIE_EPS;
 
 template <int LPL>
 struct Vec {
@@ -323,19 +328,20 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         for (int k = 0; k < LPL; ++k) F.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
       }
       float best = kInfF;
-      int bj = 0;
 #pragma unroll
-      for (int k = 0; k < LPL; ++k) {
-        const float mm = valid[k] ? F.v[k] + b.v[k] : kInfF;
-        if (mm < best) {
-          best = mm;
-          bj = k;
-        }
-      }
+      for (int k = 0; k < LPL; ++k) best = fminf(best, valid[k] ? F.v[k] + b.v[k] : kInfF);
       const float gbest = wave_min(best);
-      const unsigned long long ball = __ballot(best == gbest);
+      // Lowest label within a float32 tolerance of the minimum: min-marginals
+      // that tie exactly (plateaus of the truncated pairwise term) differ here
+      // only by rounding, and the lowest-label rule must see them as ties.
+      const float thr = gbest + kTieEps * fmaxf(1.f, fabsf(gbest));
+      int bj = LPL;
+#pragma unroll
+      for (int k = LPL - 1; k >= 0; --k)
+        if (valid[k] && F.v[k] + b.v[k] <= thr) bj = k;
+      const unsigned long long ball = __ballot(bj < LPL);
       const int first = ball ? __builtin_ctzll(ball) : 0;  // empty only for NaN costs
-      const int label = __builtin_amdgcn_readlane(lane * LPL + bj, first);
+      const int label = __builtin_amdgcn_readlane(lane * LPL + (bj < LPL ? bj : 0), first);
       if (a.gap) {
         float second = kInfF;
 #pragma unroll

@@ -61,7 +61,7 @@ class Frame:
         fr = self.tower(h, xr)
         mark("correlation")
         cost = ops.correlation_volume(h, fl, fr, self.C, self.disp_min, self.disp_step, self.D)
-        out = dict(cost=cost)
+        out = dict(cost=cost, features=(fl, fr))
         if crf:
             mark("edge_weights")
             gray = ops.rgb_to_gray(h, left) if self.color else left[0]

@@ -25,6 +25,9 @@
 
 namespace {
 constexpr double kInf = std::numeric_limits<double>::infinity();
+// Relative tolerance under which decoded min-marginals count as tied (the GPU
+// decode uses the same rule in float32).
+constexpr double kTieEps = 1e-5;
 }
 
 extern "C" int or_reflect(int i, int n) {
@@ -419,7 +422,9 @@ double minorant_impl(const double* theta, const double* w, int n, int D, double 
 }
 
 // Decode of one chain: labels = argmin of the min-marginals F_i + B_i
-// (lowest label on ties); gap = second-best minus best.
+// (lowest label among those within kTieEps of the minimum); gap = the best
+// value over the other labels minus the minimum (0 when the chosen label is
+// not the exact minimum).
 double decode_impl(const double* theta, const double* w, int n, int D, double L1, double L2,
                    int delta, int* labels, double* gap, std::vector<double>& Bh) {
   Bh.resize((long)n * D);
@@ -434,18 +439,18 @@ double decode_impl(const double* theta, const double* w, int n, int D, double L1
       minconv(F.data(), D, w[i - 1], L1, L2, delta, msg.data());
       for (int k = 0; k < D; ++k) F[k] = theta[(long)i * D + k] + msg[k] - f;
     }
-    double best = kInf, second = kInf;
+    // Lowest label whose min-marginal is within kTieEps (relative) of the
+    // minimum: min-marginals on a plateau of the truncated pairwise term tie in
+    // exact arithmetic but differ by rounding, and the lowest-label rule must
+    // see them as ties (a spatially coherent choice).
+    double best = kInf;
+    for (int k = 0; k < D; ++k) best = std::min(best, F[k] + Bh[(long)i * D + k]);
+    const double thr = best + kTieEps * std::max(1.0, std::fabs(best));
     int bl = 0;
-    for (int k = 0; k < D; ++k) {
-      const double m = F[k] + Bh[(long)i * D + k];
-      if (m < best) {
-        second = best;
-        best = m;
-        bl = k;
-      } else if (m < second) {
-        second = m;
-      }
-    }
+    while (bl < D - 1 && !(F[bl] + Bh[(long)i * D + bl] <= thr)) ++bl;
+    double second = kInf;
+    for (int k = 0; k < D; ++k)
+      if (k != bl) second = std::min(second, F[k] + Bh[(long)i * D + k]);
     labels[i] = bl;
     if (gap) gap[i] = second - best;
   }
