@@ -108,9 +108,10 @@ __device__ __forceinline__ float local_min(const Vec<LPL>& x) {
 }
 
 // near(s) = min over 0 < |s - t| < delta of x(t)  (+inf if delta == 1).
-template <int LPL>
+// DC = 2 fixes delta = 2 at compile time (no branch in the step); DC = 0 reads it.
+template <int LPL, int DC>
 __device__ __forceinline__ Vec<LPL> near_min(const Vec<LPL>& x, int delta) {
-  if (delta == 2) {  // the common case, straight-line
+  if (DC == 2 || delta == 2) {  // the common case, straight-line
     const Vec<LPL> l = shift_label(x, -1, kInfF), r = shift_label(x, +1, kInfF);
     Vec<LPL> o;
 #pragma unroll
@@ -130,9 +131,9 @@ __device__ __forceinline__ Vec<LPL> near_min(const Vec<LPL>& x, int delta) {
   return acc;
 }
 
-template <int LPL>
+template <int LPL, int DC>
 __device__ __forceinline__ Vec<LPL> near_max(const Vec<LPL>& x, int delta) {
-  if (delta == 2) {
+  if (DC == 2 || delta == 2) {
     const Vec<LPL> l = shift_label(x, -1, -kInfF), r = shift_label(x, +1, -kInfF);
     Vec<LPL> o;
 #pragma unroll
@@ -153,10 +154,10 @@ __device__ __forceinline__ Vec<LPL> near_max(const Vec<LPL>& x, int delta) {
 }
 
 // m(s) = min(U(s), near(U)(s) + w L1, min U + w L2).
-template <int LPL>
+template <int LPL, int DC>
 __device__ __forceinline__ Vec<LPL> minconv(const Vec<LPL>& U, float umin, float w, float L1, float L2,
                                             int delta) {
-  const Vec<LPL> nr = near_min(U, delta);
+  const Vec<LPL> nr = near_min<LPL, DC>(U, delta);
   const float far = umin + w * L2;
   const float wl1 = w * L1;
   Vec<LPL> m;
@@ -216,7 +217,7 @@ __device__ __forceinline__ void st_node(float* __restrict__ base, int lane, int 
 // of registers and the load for node s + P is issued right after slot s is read
 // (clamped to the last node, so it never leaves the chain).  The last partial
 // block is an unrolled tail that issues no loads.
-template <int LPL, int MODE, int P, bool FULL>
+template <int LPL, int MODE, int P, bool FULL, int DC>
 __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
   constexpr bool kCols = MODE == kColsMinorant;
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
 #pragma unroll
     for (int k = 0; k < LPL; ++k) U.v[k] = th.v[k] + bh.v[k];
     const float u = wave_min(local_min(U));
-    const Vec<LPL> m = minconv(U, u, wi, a.L1, a.L2, a.delta);
+    const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, a.L1, a.L2, a.delta);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) bh.v[k] = valid[k] ? m.v[k] - u : 0.f;
     st_node<LPL, FULL>(msg + (long)i * D, lane, D, bh);
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       if (i > 0) {
         const float wv = w_lds[i - 1];
         const float f = wave_min(local_min(F));
-        const Vec<LPL> m = minconv(F, f, wv, a.L1, a.L2, a.delta);
+        const Vec<LPL> m = minconv<LPL, DC>(F, f, wv, a.L1, a.L2, a.delta);
 #pragma unroll
         for (int k = 0; k < LPL; ++k) F.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
       }
@@ -395,11 +396,11 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       }
       kept += kGammaD * (r + bi);
       const float v = wave_min(local_min(V));
-      const Vec<LPL> m = minconv(V, v, wi, a.L1, a.L2, a.delta);
+      const Vec<LPL> m = minconv<LPL, DC>(V, v, wi, a.L1, a.L2, a.delta);
       Vec<LPL> mm;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) mm.v[k] = valid[k] ? m.v[k] : -kInfF;
-      const Vec<LPL> nmax = near_max(mm, a.delta);
+      const Vec<LPL> nmax = near_max<LPL, DC>(mm, a.delta);
       const float wl1 = wi * a.L1;
       const Vec<LPL> thn = theta_of(an, cn);
       Vec<LPL> out;
@@ -451,10 +452,12 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
 
 template <int MODE, int LPL, int P>
 static void launch_lpl(ChainArgs& a, size_t lds, hipStream_t st) {
-  if (a.D == 64 * LPL)
-    k_chain<LPL, MODE, P, true><<<a.n_chains, 64, lds, st>>>(a);
+  if (a.D == 64 * LPL && a.delta == 2)
+    k_chain<LPL, MODE, P, true, 2><<<a.n_chains, 64, lds, st>>>(a);
+  else if (a.D == 64 * LPL)
+    k_chain<LPL, MODE, P, true, 0><<<a.n_chains, 64, lds, st>>>(a);
   else
-    k_chain<LPL, MODE, P, false><<<a.n_chains, 64, lds, st>>>(a);
+    k_chain<LPL, MODE, P, false, 0><<<a.n_chains, 64, lds, st>>>(a);
 }
 
 // Prefetch depth (nodes in flight per stream) by labels per lane; the row
