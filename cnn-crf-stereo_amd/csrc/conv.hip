@@ -70,6 +70,9 @@ __global__ void k_pack_f32(const float* __restrict__ w, int K, int C, int Cs, in
 // BF16 MFMA implicit GEMM
 
 constexpr int kTH = 4;    // output rows per tile (one per wave)
+#ifndef CCS_CONV_SCHED
+#define CCS_CONV_SCHED 1
+#endif
 constexpr int kLdsBytes = 160 * 1024;
 
 // TW output pixels per row; the input tile plus one weight group fit in
@@ -181,10 +184,18 @@ __global__ __launch_bounds__(256) void k_conv_bf16(const bf16* __restrict__ x, i
     for (int ql = 0; ql < nq; ++ql) {
       bf16x8 an[MT], bn[NT];
       if (ql + 1 < nq) load_frags(q0 + ql + 1, ql + 1, an, bn);
+#if CCS_CONV_SCHED
+      // Issue all of chunk q+1's LDS reads before chunk q's MFMAs, so their
+      // latency hides behind the MFMAs instead of being scheduled just in time.
+      __builtin_amdgcn_sched_barrier(0);
+#endif
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[t], acc[m][t], 0, 0, 0);
+#if CCS_CONV_SCHED
+      __builtin_amdgcn_sched_barrier(0);
+#endif
 #pragma unroll
       for (int m = 0; m < MT; ++m) a[m] = an[m];
 #pragma unroll

@@ -217,6 +217,61 @@ __device__ __forceinline__ void st_node(float* __restrict__ base, int lane, int 
 // of registers and the load for node s + P is issued right after slot s is read
 // (clamped to the last node, so it never leaves the chain).  The last partial
 // block is an unrolled tail that issues no loads.
+// Suffix sums b_i = u_i + ... + u_{n-2} (b_{n-1} = 0) of the backward-pass
+// minima, in float64, the 64 lanes each taking a contiguous segment; returns b_0.
+// Done once after the pass so the per-node step carries no float64 work.
+__device__ double chain_offsets(const float* u, double* b, int n, int lane) {
+  const int m = n - 1;
+  const int L = (m + 63) / 64;
+  const int s0 = min(lane * L, m), s1 = min(s0 + L, m);
+  double tot = 0.0;
+  for (int j = s0; j < s1; ++j) tot += (double)u[j];
+  double incl = tot;  // sum over this lane and the lanes above it
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_down(incl, o, 64);
+    if (lane + o < 64) incl += t;
+  }
+  double acc = __shfl_down(incl, 1, 64);
+  if (lane == 63) acc = 0.0;
+  for (int j = s1 - 1; j >= s0; --j) {
+    acc += (double)u[j];
+    b[j] = acc;
+  }
+  if (lane == 0) b[m] = 0.0;
+  return __shfl(incl, 0, 64);
+}
+
+// Kept constant of the forward pass: r_0 = 0, r_{i+1} = a r_i + v_i - g b_i with
+// a = 1 - g for i < m = n - 1; returns g * sum_{i<m} (r_i + b_i) + r_m.  Each
+// lane runs its segment from r = 0 and the segments are joined by a scan of the
+// affine maps x -> a^len x + r_end.
+__device__ double chain_kept(const float* v, const double* b, int n, int lane, double g) {
+  const int m = n - 1;
+  const double a = 1.0 - g;
+  const int L = (m + 63) / 64;
+  const int s0 = min(lane * L, m), s1 = min(s0 + L, m);
+  double r = 0.0, sr = 0.0, sb = 0.0, geo = 0.0, ap = 1.0;
+  for (int j = s0; j < s1; ++j) {
+    sr += r;
+    geo += ap;
+    sb += b[j];
+    r = a * r + ((double)v[j] - g * b[j]);
+    ap *= a;
+  }
+  double A = ap, B = r;  // this lane's map, then the composition of lanes 0..lane
+  for (int o = 1; o < 64; o <<= 1) {
+    const double Ap = __shfl_up(A, o, 64), Bp = __shfl_up(B, o, 64);
+    if (lane >= o) {
+      B = A * Bp + B;
+      A = A * Ap;
+    }
+  }
+  double rin = __shfl_up(B, 1, 64);  // r at the start of this lane's segment
+  if (lane == 0) rin = 0.0;
+  const double total = wave_sum_f64(g * (sr + rin * geo + sb));
+  return total + __shfl(B, 63, 64);
+}
+
 template <int LPL, int MODE, int P, bool FULL, int DC>
 __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
@@ -243,28 +298,26 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   };
 
   // The chain's edge weights go to LDS once; each step reads its weight there.
+  // u_lds takes the per-node minima of a pass (float64 sums are formed after it).
   float* w_lds = reinterpret_cast<float*>(b_lds + n);
+  float* u_lds = w_lds + n;
   for (int i = lane; i < n - 1; i += 64) w_lds[i] = wb[(long)i * a.pns];
   __syncthreads();
 
   Vec<LPL> ra[P], rc[P], rb[P];
-  float rw[P];
 
   // ---- backward pass: step s handles i = n-2-s: Bh[i] = minconv(theta[i+1] + Bh[i+1]) - min(...)
   const int nb = n - 1;
-  auto load_b = [&](int s, Vec<LPL>& av, Vec<LPL>& cv, float& w) {
+  auto load_b = [&](int s, Vec<LPL>& av, Vec<LPL>& cv) {
     const int node = s < nb ? n - 1 - s : 1;  // clamped: beyond the chain re-read node 1
     av = ld_node<LPL, FULL>(Ab + node * nstride, lane, D, kInfF);
     if constexpr (kCols) cv = ld_node<LPL, FULL>(Cb + node * nstride, lane, D, kInfF);
-    (void)w;
   };
   Vec<LPL> bh;
 #pragma unroll
   for (int j = 0; j < LPL; ++j) bh.v[j] = 0.f;
   st_node<LPL, FULL>(msg + (long)(n - 1) * D, lane, D, bh);
-  double bsum = 0.0;
-  b_lds[n - 1] = 0.0;
-  auto step_b = [&](int s, const Vec<LPL>& th, float) {
+  auto step_b = [&](int s, const Vec<LPL>& th) {
     const int i = n - 2 - s;
     const float wi = w_lds[i];
     Vec<LPL> U;
@@ -275,29 +328,28 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
 #pragma unroll
     for (int k = 0; k < LPL; ++k) bh.v[k] = valid[k] ? m.v[k] - u : 0.f;
     st_node<LPL, FULL>(msg + (long)i * D, lane, D, bh);
-    bsum += (double)u;
-    b_lds[i] = bsum;  // every lane writes the same value
+    u_lds[i] = u;  // every lane writes the same value
   };
   if (nb > 0) {
 #pragma unroll
-    for (int j = 0; j < P; ++j) load_b(j, ra[j], rc[j], rw[j]);
+    for (int j = 0; j < P; ++j) load_b(j, ra[j], rc[j]);
   }
   int s0 = 0;
   for (; s0 + P <= nb; s0 += P) {
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const Vec<LPL> th = theta_of(ra[j], rc[j]);
-      const float wi = rw[j];
       // Keep "read slot j" before "refill slot j": the refill then reuses the
       // slot's registers and the ring needs no copies at the loop back-edge.
       __builtin_amdgcn_sched_barrier(0);
-      load_b(s0 + j + P, ra[j], rc[j], rw[j]);
-      step_b(s0 + j, th, wi);
+      load_b(s0 + j + P, ra[j], rc[j]);
+      step_b(s0 + j, th);
     }
   }
 #pragma unroll
   for (int j = 0; j < P; ++j)
-    if (s0 + j < nb) step_b(s0 + j, theta_of(ra[j], rc[j]), rw[j]);
+    if (s0 + j < nb) step_b(s0 + j, theta_of(ra[j], rc[j]));
+  const double bsum = chain_offsets(u_lds, b_lds, n, lane);  // b_0 = sum of all minima
 
   // bh holds Bh[0]; theta[0] is read directly.
   const Vec<LPL> a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
@@ -312,14 +364,13 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
 
   if constexpr (MODE == kRowsDecode) {
     // Forward min-sum F, label = argmin(F + Bh) (lowest label on ties).
-    auto load_d = [&](int i, Vec<LPL>& bv, Vec<LPL>& av, float& w) {
+    auto load_d = [&](int i, Vec<LPL>& bv, Vec<LPL>& av) {
       const int node = i < n ? i : n - 1;
       bv = ld_node<LPL, FULL>(msg + (long)node * D, lane, D, 0.f);
       av = ld_node<LPL, FULL>(Ab + node * nstride, lane, D, kInfF);
-      (void)w;
     };
     Vec<LPL> F = th0;
-    auto step_d = [&](int i, const Vec<LPL>& b, const Vec<LPL>& an, float) {
+    auto step_d = [&](int i, const Vec<LPL>& b, const Vec<LPL>& an) {
       const long p = pbase + (long)i * a.pns;
       if (i > 0) {
         const float wv = w_lds[i - 1];
@@ -356,37 +407,35 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       if (lane == 0) a.labels[p] = label;
     };
 #pragma unroll
-    for (int j = 0; j < P; ++j) load_d(j, rb[j], ra[j], rw[j]);
+    for (int j = 0; j < P; ++j) load_d(j, rb[j], ra[j]);
     int i0 = 0;
     for (; i0 + P <= n; i0 += P) {
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         const Vec<LPL> b = rb[j];
         const Vec<LPL> an = ra[j];
-        const float wv = rw[j];
         __builtin_amdgcn_sched_barrier(0);
-        load_d(i0 + j + P, rb[j], ra[j], rw[j]);
-        step_d(i0 + j, b, an, wv);
+        load_d(i0 + j + P, rb[j], ra[j]);
+        step_d(i0 + j, b, an);
       }
     }
 #pragma unroll
     for (int j = 0; j < P; ++j)
-      if (i0 + j < n) step_d(i0 + j, rb[j], ra[j], rw[j]);
+      if (i0 + j < n) step_d(i0 + j, rb[j], ra[j]);
     if (lane == 0) a.lam[c] = Lambda;
   } else {
     // Forward pass: keep g_i = gamma (R + Bh) + excess, pass V = R - gamma (R + Bh) on.
-    auto load_f = [&](int i, Vec<LPL>& bv, Vec<LPL>& av, Vec<LPL>& cv, float& w) {
+    auto load_f = [&](int i, Vec<LPL>& bv, Vec<LPL>& av, Vec<LPL>& cv) {
       const int node = i < nb ? i : nb - 1;
       bv = ld_node<LPL, FULL>(msg + (long)node * D, lane, D, 0.f);
       av = ld_node<LPL, FULL>(Ab + (node + 1) * nstride, lane, D, kInfF);
       if constexpr (kCols) cv = ld_node<LPL, FULL>(Cb + (node + 1) * nstride, lane, D, kInfF);
-      (void)w;
     };
     Vec<LPL> R = th0;
     Vec<LPL> a_cur = a0;
-    double r = 0.0, kept = 0.0;
-    auto step_f = [&](int i, const Vec<LPL>& b, const Vec<LPL>& an, const Vec<LPL>& cn, float) {
-      const double bi = b_lds[i];
+    // The float64 kept constant (the chain minimum this pass leaves behind) is
+    // formed from the per-node minima v_i after the pass: chain_kept().
+    auto step_f = [&](int i, const Vec<LPL>& b, const Vec<LPL>& an, const Vec<LPL>& cn) {
       const float wi = w_lds[i];
       Vec<LPL> gt, V;
 #pragma unroll
@@ -394,8 +443,8 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         gt.v[k] = valid[k] ? kGamma * (R.v[k] + b.v[k]) : 0.f;
         V.v[k] = valid[k] ? R.v[k] - gt.v[k] : kInfF;
       }
-      kept += kGammaD * (r + bi);
       const float v = wave_min(local_min(V));
+      u_lds[i] = v;  // every lane writes the same value
       const Vec<LPL> m = minconv<LPL, DC>(V, v, wi, a.L1, a.L2, a.delta);
       Vec<LPL> mm;
 #pragma unroll
@@ -412,12 +461,11 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         R.v[k] = valid[k] ? thn.v[k] + m.v[k] - v : kInfF;
       }
       st_node<LPL, FULL>(Aw + i * nstride, lane, D, out);
-      r = (double)v + (1.0 - kGammaD) * r - kGammaD * bi;
       a_cur = an;
     };
     if (nb > 0) {
 #pragma unroll
-      for (int j = 0; j < P; ++j) load_f(j, rb[j], ra[j], rc[j], rw[j]);
+      for (int j = 0; j < P; ++j) load_f(j, rb[j], ra[j], rc[j]);
     }
     int i0 = 0;
     for (; i0 + P <= nb; i0 += P) {
@@ -426,23 +474,22 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         const Vec<LPL> b = rb[j];
         const Vec<LPL> an = ra[j];
         const Vec<LPL> cn = rc[j];
-        const float wi = rw[j];
         __builtin_amdgcn_sched_barrier(0);
-        load_f(i0 + j + P, rb[j], ra[j], rc[j], rw[j]);
-        step_f(i0 + j, b, an, cn, wi);
+        load_f(i0 + j + P, rb[j], ra[j], rc[j]);
+        step_f(i0 + j, b, an, cn);
       }
     }
 #pragma unroll
     for (int j = 0; j < P; ++j)
-      if (i0 + j < nb) step_f(i0 + j, rb[j], ra[j], rc[j], rw[j]);
+      if (i0 + j < nb) step_f(i0 + j, rb[j], ra[j], rc[j]);
     {
       Vec<LPL> out;
 #pragma unroll
       for (int k = 0; k < LPL; ++k)
         out.v[k] = MODE == kRowsMinorant ? a_cur.v[k] - R.v[k] : a_cur.v[k] + R.v[k];
       st_node<LPL, FULL>(Aw + (n - 1) * nstride, lane, D, out);
-      kept += r;
     }
+    const double kept = chain_kept(u_lds, b_lds, n, lane, kGammaD);
     if (lane == 0) {
       a.lam[c] = Lambda;
       a.kept[c] = kept;
@@ -450,14 +497,22 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   }
 }
 
+template <int LPL, int MODE, int P, bool FULL, int DC>
+static void launch_one(ChainArgs& a, size_t lds, hipStream_t st) {
+  if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
+    (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  k_chain<LPL, MODE, P, FULL, DC><<<a.n_chains, 64, lds, st>>>(a);
+}
+
 template <int MODE, int LPL, int P>
 static void launch_lpl(ChainArgs& a, size_t lds, hipStream_t st) {
   if (a.D == 64 * LPL && a.delta == 2)
-    k_chain<LPL, MODE, P, true, 2><<<a.n_chains, 64, lds, st>>>(a);
+    launch_one<LPL, MODE, P, true, 2>(a, lds, st);
   else if (a.D == 64 * LPL)
-    k_chain<LPL, MODE, P, true, 0><<<a.n_chains, 64, lds, st>>>(a);
+    launch_one<LPL, MODE, P, true, 0>(a, lds, st);
   else
-    k_chain<LPL, MODE, P, false, 0><<<a.n_chains, 64, lds, st>>>(a);
+    launch_one<LPL, MODE, P, false, 0>(a, lds, st);
 }
 
 // Prefetch depth (nodes in flight per stream) by labels per lane; the row
@@ -472,9 +527,9 @@ static void launch_lpl(ChainArgs& a, size_t lds, hipStream_t st) {
 
 template <int MODE>
 static int launch_mode(ChainArgs& a, hipStream_t st) {
-  const size_t lds = (sizeof(double) + sizeof(float)) * (size_t)a.n;
+  const size_t lds = (sizeof(double) + 2 * sizeof(float)) * (size_t)a.n;
   constexpr int P2 = MODE == kColsMinorant ? CCS_CRF_P_COLS : CCS_CRF_P_ROWS;
-  if (a.D <= 64) launch_lpl<MODE, 1, 2 * P2>(a, lds, st);
+  if (a.D <= 64) launch_lpl<MODE, 1, P2>(a, lds, st);
   else if (a.D <= 128) launch_lpl<MODE, 2, P2>(a, lds, st);
   else launch_lpl<MODE, 4, (P2 / 2 > 4 ? P2 / 2 : 4)>(a, lds, st);
   CCS_LAUNCH_CHECK();
