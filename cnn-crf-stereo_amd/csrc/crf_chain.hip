@@ -21,8 +21,10 @@
 // GPU mapping: one wavefront per chain, label s = lane * LPL + j held in VGPRs,
 // DPP wave_shr/wave_shl for the label neighbours and a DPP tree for the minimum
 // over labels.  The volume streams through HBM once per pass (node i is D
-// contiguous floats), backward messages go to a scratch volume and the per-node
-// float64 offsets to LDS.
+// contiguous floats), backward messages go to a scratch volume, and the
+// per-node minima go to LDS; their float64 sums are formed by wave scans after
+// each pass.  The row decode runs its two message passes on two wavefronts at
+// once (k_decode2).
 //
 // This file holds the chain sweeps only.  It is compiled with -fno-honor-nans:
 // the min/max trees then run as fused DPP instructions without NaN-quieting
@@ -211,12 +213,6 @@ __device__ __forceinline__ void st_node(float* __restrict__ base, int lane, int 
   }
 }
 
-// Pipelined chain sweep.  Every input stream of a pass (unaries, cost, backward
-// messages, edge weights) runs P nodes ahead of the node being processed: the
-// main loop handles blocks of P nodes, unrolled, so ring slot j is a fixed set
-// of registers and the load for node s + P is issued right after slot s is read
-// (clamped to the last node, so it never leaves the chain).  The last partial
-// block is an unrolled tail that issues no loads.
 // Suffix sums b_i = u_i + ... + u_{n-2} (b_{n-1} = 0) of the backward-pass
 // minima, in float64, the 64 lanes each taking a contiguous segment; returns b_0.
 // Done once after the pass so the per-node step carries no float64 work.
@@ -272,6 +268,12 @@ __device__ double chain_kept(const float* v, const double* b, int n, int lane, d
   return total + __shfl(B, 63, 64);
 }
 
+// Pipelined chain sweep.  Every input stream of a pass (unaries, cost, backward
+// messages, edge weights) runs P nodes ahead of the node being processed: the
+// main loop handles blocks of P nodes, unrolled, so ring slot j is a fixed set
+// of registers and the load for node s + P is issued right after slot s is read
+// (clamped to the last node, so it never leaves the chain).  The last partial
+// block is an unrolled tail that issues no loads.
 template <int LPL, int MODE, int P, bool FULL, int DC>
 __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
@@ -362,68 +364,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   for (int k = 0; k < LPL; ++k) M0.v[k] = th0.v[k] + bh.v[k];
   const double Lambda = bsum + (double)wave_min(local_min(M0));
 
-  if constexpr (MODE == kRowsDecode) {
-    // Forward min-sum F, label = argmin(F + Bh) (lowest label on ties).
-    auto load_d = [&](int i, Vec<LPL>& bv, Vec<LPL>& av) {
-      const int node = i < n ? i : n - 1;
-      bv = ld_node<LPL, FULL>(msg + (long)node * D, lane, D, 0.f);
-      av = ld_node<LPL, FULL>(Ab + node * nstride, lane, D, kInfF);
-    };
-    Vec<LPL> F = th0;
-    auto step_d = [&](int i, const Vec<LPL>& b, const Vec<LPL>& an) {
-      const long p = pbase + (long)i * a.pns;
-      if (i > 0) {
-        const float wv = w_lds[i - 1];
-        const float f = wave_min(local_min(F));
-        const Vec<LPL> m = minconv<LPL, DC>(F, f, wv, a.L1, a.L2, a.delta);
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) F.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
-      }
-      float best = kInfF;
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) best = fminf(best, valid[k] ? F.v[k] + b.v[k] : kInfF);
-      const float gbest = wave_min(best);
-      // Lowest label within a float32 tolerance of the minimum: min-marginals
-      // that tie exactly (plateaus of the truncated pairwise term) differ here
-      // only by rounding, and the lowest-label rule must see them as ties.
-      const float thr = gbest + kTieEps * fmaxf(1.f, fabsf(gbest));
-      int bj = LPL;
-#pragma unroll
-      for (int k = LPL - 1; k >= 0; --k)
-        if (valid[k] && F.v[k] + b.v[k] <= thr) bj = k;
-      const unsigned long long ball = __ballot(bj < LPL);
-      const int first = ball ? __builtin_ctzll(ball) : 0;  // empty only for NaN costs
-      const int label = __builtin_amdgcn_readlane(lane * LPL + (bj < LPL ? bj : 0), first);
-      if (a.gap) {
-        float second = kInfF;
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) {
-          const float mm = valid[k] ? F.v[k] + b.v[k] : kInfF;
-          if (lane * LPL + k != label) second = fminf(second, mm);
-        }
-        const float gsecond = wave_min(second);
-        if (lane == 0) a.gap[p] = gsecond - gbest;
-      }
-      if (lane == 0) a.labels[p] = label;
-    };
-#pragma unroll
-    for (int j = 0; j < P; ++j) load_d(j, rb[j], ra[j]);
-    int i0 = 0;
-    for (; i0 + P <= n; i0 += P) {
-#pragma unroll
-      for (int j = 0; j < P; ++j) {
-        const Vec<LPL> b = rb[j];
-        const Vec<LPL> an = ra[j];
-        __builtin_amdgcn_sched_barrier(0);
-        load_d(i0 + j + P, rb[j], ra[j]);
-        step_d(i0 + j, b, an);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < P; ++j)
-      if (i0 + j < n) step_d(i0 + j, rb[j], ra[j]);
-    if (lane == 0) a.lam[c] = Lambda;
-  } else {
+  {
     // Forward pass: keep g_i = gamma (R + Bh) + excess, pass V = R - gamma (R + Bh) on.
     auto load_f = [&](int i, Vec<LPL>& bv, Vec<LPL>& av, Vec<LPL>& cv) {
       const int node = i < nb ? i : nb - 1;
@@ -497,12 +438,174 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   }
 }
 
+// Sum of u_0 .. u_{n-2} in float64 (64 lanes, contiguous segments).
+__device__ double chain_total(const float* u, int n, int lane) {
+  const int m = n - 1;
+  const int L = (m + 63) / 64;
+  const int s0 = min(lane * L, m), s1 = min(s0 + L, m);
+  double tot = 0.0;
+  for (int j = s0; j < s1; ++j) tot += (double)u[j];
+  return wave_sum_f64(tot);
+}
+
+// Runs step(node, A_node+aoff, msg_node) for cnt nodes first, first + dir, ...,
+// with the unaries (and, with MSG, the message buffer) P nodes ahead in a
+// register ring, as in k_chain.
+template <int LPL, bool FULL, int P, bool MSG, typename Step>
+__device__ __forceinline__ void ring_pass(const float* __restrict__ Ab, long nstride, const float* msg, int D,
+                                          int lane, int first, int cnt, int dir, int aoff, Step&& step) {
+  if (cnt <= 0) return;
+  Vec<LPL> ra[P], rm[P];
+  auto load = [&](int s, Vec<LPL>& av, Vec<LPL>& mv) {
+    const int i = first + dir * (s < cnt ? s : cnt - 1);
+    av = ld_node<LPL, FULL>(Ab + (i + aoff) * nstride, lane, D, kInfF);
+    if constexpr (MSG) mv = ld_node<LPL, FULL>(msg + (long)i * D, lane, D, 0.f);
+  };
+#pragma unroll
+  for (int j = 0; j < P; ++j) load(j, ra[j], rm[j]);
+  int s0 = 0;
+  for (; s0 + P <= cnt; s0 += P) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const Vec<LPL> av = ra[j], mv = rm[j];
+      __builtin_amdgcn_sched_barrier(0);
+      load(s0 + j + P, ra[j], rm[j]);
+      step(first + dir * (s0 + j), av, mv);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j)
+    if (s0 + j < cnt) step(first + dir * (s0 + j), ra[j], rm[j]);
+}
+
+// Row decode with both message passes at once: wave 0 runs the forward
+// messages F from node 0, wave 1 the backward messages Bh from node n-1.  Until
+// the midpoint each stores its messages; after one workgroup barrier each
+// decodes the nodes on the far side, where the other wave's messages already
+// sit in the buffer (wave 0 the right half, wave 1 the left half).  F_i + Bh_i
+// and the decision rule are those of the single-wave decode, so the labels are
+// the same; the chain latency is n steps instead of 2n.
+template <int LPL, int P, bool FULL, int DC>
+__global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
+  extern __shared__ double lds2[];
+  const int c = blockIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform branch
+  const int lane = threadIdx.x & 63;
+  const int n = a.n, D = a.D;
+  const long pbase = (long)c * a.pcs;
+  const long nstride = a.pns * (long)D;
+  const float* __restrict__ Ab = a.A + pbase * D;
+  const float* __restrict__ wb = a.w + pbase;
+  float* msg = a.msg + (long)c * n * D;
+  float* w_lds = reinterpret_cast<float*>(lds2);
+  float* u_lds = w_lds + n;
+  bool valid[LPL];
+#pragma unroll
+  for (int j = 0; j < LPL; ++j) valid[j] = FULL || lane * LPL + j < D;
+  for (int i = threadIdx.x; i < n - 1; i += 128) w_lds[i] = wb[(long)i * a.pns];
+  __syncthreads();
+  const int mid = n / 2;
+
+  Vec<LPL> X;  // wave 0: F_i, wave 1: Bh_i
+  auto fwd = [&](int i, const Vec<LPL>& an) {  // F_i = A_i + minconv(F_{i-1}) - min F_{i-1}
+    const float f = wave_min(local_min(X));
+    const Vec<LPL> m = minconv<LPL, DC>(X, f, w_lds[i - 1], a.L1, a.L2, a.delta);
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
+  };
+  auto bwd = [&](int i, const Vec<LPL>& an1) {  // Bh_i from Bh_{i+1} and A_{i+1}
+    Vec<LPL> U;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) U.v[k] = (valid[k] ? an1.v[k] : kInfF) + X.v[k];
+    const float u = wave_min(local_min(U));
+    const Vec<LPL> m = minconv<LPL, DC>(U, u, w_lds[i], a.L1, a.L2, a.delta);
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? m.v[k] - u : 0.f;
+    u_lds[i] = u;  // every lane writes the same value
+  };
+  auto decode = [&](int i, const Vec<LPL>& F, const Vec<LPL>& b) {
+    const long p = pbase + (long)i * a.pns;
+    float best = kInfF;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) best = fminf(best, valid[k] ? F.v[k] + b.v[k] : kInfF);
+    const float gbest = wave_min(best);
+    const float thr = gbest + kTieEps * fmaxf(1.f, fabsf(gbest));
+    int bj = LPL;
+#pragma unroll
+    for (int k = LPL - 1; k >= 0; --k)
+      if (valid[k] && F.v[k] + b.v[k] <= thr) bj = k;
+    const unsigned long long ball = __ballot(bj < LPL);
+    const int first = ball ? __builtin_ctzll(ball) : 0;  // empty only for NaN costs
+    const int label = __builtin_amdgcn_readlane(lane * LPL + (bj < LPL ? bj : 0), first);
+    if (a.gap) {
+      float second = kInfF;
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) {
+        const float mm = valid[k] ? F.v[k] + b.v[k] : kInfF;
+        if (lane * LPL + k != label) second = fminf(second, mm);
+      }
+      const float gsecond = wave_min(second);
+      if (lane == 0) a.gap[p] = gsecond - gbest;
+    }
+    if (lane == 0) a.labels[p] = label;
+  };
+
+  // Phase 1: each wave walks to the midpoint, storing its messages.
+  if (wv == 0) {
+    X = ld_node<LPL, FULL>(Ab, lane, D, kInfF);  // F_0 = A_0
+    if (mid > 0) st_node<LPL, FULL>(msg, lane, D, X);
+    ring_pass<LPL, FULL, P, false>(Ab, nstride, msg, D, lane, 1, mid - 1, 1, 0,
+                                   [&](int i, const Vec<LPL>& an, const Vec<LPL>&) {
+                                     fwd(i, an);
+                                     st_node<LPL, FULL>(msg + (long)i * D, lane, D, X);
+                                   });
+  } else {
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) X.v[k] = 0.f;  // Bh_{n-1} = 0
+    st_node<LPL, FULL>(msg + (long)(n - 1) * D, lane, D, X);
+    ring_pass<LPL, FULL, P, false>(Ab, nstride, msg, D, lane, n - 2, n - 1 - mid, -1, 1,
+                                   [&](int i, const Vec<LPL>& an1, const Vec<LPL>&) {
+                                     bwd(i, an1);
+                                     st_node<LPL, FULL>(msg + (long)i * D, lane, D, X);
+                                   });
+  }
+  __threadfence_block();
+  __syncthreads();  // executed by both waves: the other half's messages are now visible
+  // Phase 2: each wave continues into the other half and decodes it.
+  if (wv == 0) {
+    ring_pass<LPL, FULL, P, true>(Ab, nstride, msg, D, lane, mid, n - mid, 1, 0,
+                                  [&](int i, const Vec<LPL>& an, const Vec<LPL>& bh) {
+                                    if (i > 0) fwd(i, an);
+                                    decode(i, X, bh);
+                                  });
+  } else {
+    ring_pass<LPL, FULL, P, true>(Ab, nstride, msg, D, lane, mid - 1, mid, -1, 1,
+                                  [&](int i, const Vec<LPL>& an1, const Vec<LPL>& F) {
+                                    bwd(i, an1);
+                                    decode(i, F, X);
+                                  });
+    // Chain minimum: sum of the backward minima + min(A_0 + Bh_0).
+    const Vec<LPL> a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
+    Vec<LPL> M0;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) M0.v[k] = a0.v[k] + X.v[k];
+    const double Lambda = chain_total(u_lds, n, lane) + (double)wave_min(local_min(M0));
+    if (lane == 0) a.lam[c] = Lambda;
+  }
+}
+
 template <int LPL, int MODE, int P, bool FULL, int DC>
 static void launch_one(ChainArgs& a, size_t lds, hipStream_t st) {
-  if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
-    (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  k_chain<LPL, MODE, P, FULL, DC><<<a.n_chains, 64, lds, st>>>(a);
+  if constexpr (MODE != kRowsDecode)
+    if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
+      (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if constexpr (MODE == kRowsDecode) {
+    const size_t lds2 = 2 * sizeof(float) * (size_t)a.n;
+    k_decode2<LPL, P, FULL, DC><<<a.n_chains, 128, lds2, st>>>(a);
+  } else {
+    k_chain<LPL, MODE, P, FULL, DC><<<a.n_chains, 64, lds, st>>>(a);
+  }
 }
 
 template <int MODE, int LPL, int P>
