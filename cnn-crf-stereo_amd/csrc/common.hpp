@@ -138,6 +138,25 @@ __device__ __forceinline__ float lane_next(float x, float fill) {
   return dpp_f<0x130, 0xf, 0xf, false>(fill, x);  // wave_shl:1
 }
 
+// Global -> LDS staging of n 16-byte vectors: each thread issues BATCH loads
+// before it stores any of them, so a tile costs one memory latency per batch
+// instead of one per vector.  load(v) is called with v clamped to [0, n) and
+// returns the vector (zero where the source is outside the image); store(v, x)
+// runs for v < n only.
+template <int BATCH, typename Load, typename Store>
+__device__ __forceinline__ void batched_copy16(int n, int tid, int nthreads, Load&& load, Store&& store) {
+  for (int base = 0; base < n; base += BATCH * nthreads) {
+    u32x4 r[BATCH];
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) r[i] = load(min(base + tid + i * nthreads, n - 1));
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+      const int v = base + tid + i * nthreads;
+      if (v < n) store(v, r[i]);
+    }
+  }
+}
+
 __device__ __forceinline__ double wave_sum_f64(double x) {
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
   return x;

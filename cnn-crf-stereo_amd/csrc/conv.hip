@@ -137,15 +137,25 @@ __global__ __launch_bounds__(256) void k_conv_bf16(const bf16* __restrict__ x, i
   u32x4 wr[Cfg::B_VPT];
   CCS_B_LOAD(0);
   {
+    // All of this thread's input-tile vectors are loaded before any is stored,
+    // so the tile costs one memory latency rather than one per vector.
     constexpr int vec_per_row = Cfg::AC * CG;
-    for (int v = tid; v < Cfg::a_vec; v += 256) {
+    constexpr int A_VPT = (Cfg::a_vec + 255) / 256;
+    u32x4 ar[A_VPT];
+#pragma unroll
+    for (int i = 0; i < A_VPT; ++i) {
+      const int v = min(tid + 256 * i, Cfg::a_vec - 1);
       const int r = v / vec_per_row, rem = v % vec_per_row;
       const int px = rem / CG, cg = rem % CG;
       const int hi = h0 + r, wi = w0 + px;
-      const bool ok = hi < Hin && wi < Win;
       const u32x4 val = *reinterpret_cast<const u32x4*>(
           x + (((long)min(hi, Hin - 1) * Win + min(wi, Win - 1)) * Cs + cg * 8));
-      reinterpret_cast<u32x4*>(As)[v] = ok ? val : u32x4{0u, 0u, 0u, 0u};
+      ar[i] = (hi < Hin && wi < Win) ? val : u32x4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int i = 0; i < A_VPT; ++i) {
+      const int v = tid + 256 * i;
+      if (v < Cfg::a_vec) reinterpret_cast<u32x4*>(As)[v] = ar[i];
     }
   }
   CCS_B_STORE(0);
@@ -277,16 +287,26 @@ __global__ __launch_bounds__(256) void k_conv_f32(const float* __restrict__ x, i
   const int wrow = wave >> 1, wcol = (wave & 1) * 32;
   const int w0 = blockIdx.x * kTWf, h0 = blockIdx.y * kTHf;
   {
-    constexpr int per_row = Cfg::AC * Cs;
-    constexpr int total = Cfg::AR * per_row;
-    for (int v = tid; v < total; v += 256) {
-      const int r = v / per_row, rem = v % per_row;
-      const int px = rem / Cs, c = rem % Cs;
-      const int hi = h0 + r, wi = w0 + px;
-      float val = 0.f;
-      if (hi < Hin && wi < Win) val = x[((long)hi * Win + wi) * Cs + c];
-      As[(r * Cfg::AC + px) * CP + c] = val;
-    }
+    // 16-byte loads (Cs is a multiple of 8), scalar stores into the odd LDS pitch.
+    constexpr int vpp = Cs / 4;  // float4 per pixel
+    constexpr int per_row = Cfg::AC * vpp;
+    batched_copy16<9>(
+        Cfg::AR * per_row, tid, 256,
+        [&](int v) {
+          const int r = v / per_row, rem = v % per_row;
+          const int px = rem / vpp, c4 = rem % vpp;
+          const int hi = h0 + r, wi = w0 + px;
+          const u32x4 val = *reinterpret_cast<const u32x4*>(
+              x + ((long)min(hi, Hin - 1) * Win + min(wi, Win - 1)) * Cs + 4 * c4);
+          return (hi < Hin && wi < Win) ? val : u32x4{0u, 0u, 0u, 0u};
+        },
+        [&](int v, const u32x4& val) {
+          const int r = v / per_row, rem = v % per_row;
+          const int px = rem / vpp, c4 = rem % vpp;
+          float* d = As + (r * Cfg::AC + px) * CP + 4 * c4;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) d[e] = __uint_as_float(val[e]);
+        });
   }
   f32x4 acc[2][NT];
 #pragma unroll
@@ -299,10 +319,10 @@ __global__ __launch_bounds__(256) void k_conv_f32(const float* __restrict__ x, i
     const int nq = (Cfg::NQ - q0) < Cfg::QG ? (Cfg::NQ - q0) : Cfg::QG;
     __syncthreads();
     {
-      const int nvec = nq * Cfg::chunk_bytes / 16;
-      const float4* src = reinterpret_cast<const float4*>(wpk + (long)q0 * 4 * Cfg::NP);
-      float4* dst = reinterpret_cast<float4*>(Bs);
-      for (int v = tid; v < nvec; v += 256) dst[v] = src[v];
+      const u32x4* src = reinterpret_cast<const u32x4*>(wpk + (long)q0 * 4 * Cfg::NP);
+      u32x4* dst = reinterpret_cast<u32x4*>(Bs);
+      batched_copy16<8>(nq * Cfg::chunk_bytes / 16, tid, 256, [&](int v) { return src[v]; },
+                        [&](int v, const u32x4& val) { dst[v] = val; });
     }
     __syncthreads();
     for (int ql = 0; ql < nq; ++ql) {

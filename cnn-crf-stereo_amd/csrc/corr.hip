@@ -62,30 +62,37 @@ __global__ __launch_bounds__(256) void k_corr_volume(CorrArgs a) {
   const T* f0 = reinterpret_cast<const T*>(a.f0) + (long)h * a.W * Cs;
   const T* f1 = reinterpret_cast<const T*>(a.f1) + (long)h * a.W * Cs;
   const int wbase = w0 + a.o_lo;  // first f1 column of the window
-  if constexpr (sizeof(T) == 2) {
-    const int cg = Cs / 8;
-    for (int v = tid; v < TW * cg; v += blockDim.x) {
-      const int px = v / cg, g = v % cg, wi = w0 + px;
-      uint4 val = make_uint4(0, 0, 0, 0);
-      if (wi < a.W) val = *reinterpret_cast<const uint4*>(f0 + (long)wi * Cs + 8 * g);
-      *reinterpret_cast<uint4*>(F0 + px * P + 8 * g) = val;
+  // Both feature rows go to LDS in batches of 16-byte loads (one memory latency
+  // per batch); columns outside the image read as zero.
+  const int nthr = blockDim.x;
+  const int vpp = Cs * (int)sizeof(T) / 16;  // 16-byte vectors per pixel
+  auto fetch = [&](const T* row, int wi) -> const u32x4* {
+    return reinterpret_cast<const u32x4*>(row + (long)min(max(wi, 0), a.W - 1) * Cs);
+  };
+  auto put = [&](T* dst, int px, int g, const u32x4& val) {
+    if constexpr (sizeof(T) == 2) {
+      *reinterpret_cast<u32x4*>(dst + px * P + 8 * g) = val;
+    } else {  // odd float pitch: scalar stores
+#pragma unroll
+      for (int e = 0; e < 4; ++e) dst[px * P + 4 * g + e] = __uint_as_float(val[e]);
     }
-    for (int v = tid; v < a.win * cg; v += blockDim.x) {
-      const int px = v / cg, g = v % cg, wi = wbase + px;
-      uint4 val = make_uint4(0, 0, 0, 0);
-      if (wi >= 0 && wi < a.W) val = *reinterpret_cast<const uint4*>(f1 + (long)wi * Cs + 8 * g);
-      *reinterpret_cast<uint4*>(F1 + px * P + 8 * g) = val;
-    }
-  } else {
-    for (int v = tid; v < TW * Cs; v += blockDim.x) {
-      const int px = v / Cs, c = v % Cs, wi = w0 + px;
-      F0[px * P + c] = wi < a.W ? f0[(long)wi * Cs + c] : 0.f;
-    }
-    for (int v = tid; v < a.win * Cs; v += blockDim.x) {
-      const int px = v / Cs, c = v % Cs, wi = wbase + px;
-      F1[px * P + c] = (wi >= 0 && wi < a.W) ? f1[(long)wi * Cs + c] : 0.f;
-    }
-  }
+  };
+  batched_copy16<8>(
+      TW * vpp, tid, nthr,
+      [&](int v) {
+        const int px = v / vpp, g = v % vpp, wi = w0 + px;
+        const u32x4 val = fetch(f0, wi)[g];
+        return wi < a.W ? val : u32x4{0u, 0u, 0u, 0u};
+      },
+      [&](int v, const u32x4& val) { put(F0, v / vpp, v % vpp, val); });
+  batched_copy16<8>(
+      a.win * vpp, tid, nthr,
+      [&](int v) {
+        const int px = v / vpp, g = v % vpp, wi = wbase + px;
+        const u32x4 val = fetch(f1, wi)[g];
+        return (wi >= 0 && wi < a.W) ? val : u32x4{0u, 0u, 0u, 0u};
+      },
+      [&](int v, const u32x4& val) { put(F1, v / vpp, v % vpp, val); });
   __syncthreads();
 
   f32x4 acc[NTM];
