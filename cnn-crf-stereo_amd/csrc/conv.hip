@@ -103,8 +103,8 @@ struct Bf16Cfg {
 // the packed weights stream through LDS in groups, the next group travelling
 // in registers while the current one feeds the MFMAs.  Inside a group the A/B
 // fragments of chunk q+1 are read from LDS while chunk q's MFMAs issue.
-template <int KH, int KW, int CG, int NT, int TW, int LDS_BUDGET>
-__global__ __launch_bounds__(256) void k_conv_bf16(const bf16* __restrict__ x, int Hin, int Win,
+template <int KH, int KW, int CG, int NT, int TW, int LDS_BUDGET, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_conv_bf16(const bf16* __restrict__ x, int Hin, int Win,
                                                    const bf16* __restrict__ wpk, const float* __restrict__ bias,
                                                    int K, int act, bf16* __restrict__ y, int Ho, int Wo) {
   using Cfg = Bf16Cfg<KH, KW, CG, NT, TW, LDS_BUDGET>;
@@ -486,31 +486,35 @@ struct ConvShape {
   int H, W, C, K, KH, KW;
 };
 
-// Tile width and LDS budget per workgroup of the bf16 kernels (two workgroups per CU).
-#ifndef CCS_BF16_TW
-#define CCS_BF16_TW 32
-#endif
-#ifndef CCS_BF16_LDS
-#define CCS_BF16_LDS (80 * 1024)
-#endif
-constexpr int kBf16TW = CCS_BF16_TW;
-constexpr int kBf16Lds = CCS_BF16_LDS;
+// Tile width, LDS budget per workgroup and minimum waves per SIMD of the bf16
+// kernels, per filter size, as measured on 1242 x 375 x 100-channel layers:
+// 3x3 -- 32-pixel tiles, two 80 KB workgroups per CU (0.171 ms per layer;
+// 16-pixel tiles at three waves per SIMD: 0.192 ms); 2x2 -- 16-pixel tiles,
+// three 52000-byte workgroups per CU (0.109 ms; 32-pixel tiles: 0.123 ms).
+template <int KH>
+struct Bf16Tile {
+  static constexpr int TW = 32, LDS = 80 * 1024, MINW = 1;
+};
+template <>
+struct Bf16Tile<2> {
+  static constexpr int TW = 16, LDS = 52000, MINW = 3;
+};
 
 template <int KH, int KW, int CG, int NT>
 static bool try_bf16(const ConvShape& s, const void* x, const void* wpk, const float* b, int act, void* y,
                      hipStream_t st) {
   if (s.KH != KH || s.KW != KW || feature_stride(s.C) != CG * 8 || (s.K + 15) / 16 != NT) return false;
-  using Cfg = Bf16Cfg<KH, KW, CG, NT, kBf16TW, kBf16Lds>;
+  using T = Bf16Tile<KH>;
+  using Cfg = Bf16Cfg<KH, KW, CG, NT, T::TW, T::LDS>;
+  const auto kern = k_conv_bf16<KH, KW, CG, NT, T::TW, T::LDS, T::MINW>;
   const int Ho = s.H - KH + 1, Wo = s.W - KW + 1;
   static bool configured = false;
   if (!configured) {
-    hipFuncSetAttribute((const void*)k_conv_bf16<KH, KW, CG, NT, kBf16TW, kBf16Lds>,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
     configured = true;
   }
-  k_conv_bf16<KH, KW, CG, NT, kBf16TW, kBf16Lds>
-      <<<dim3(ceil_div(Wo, kBf16TW), ceil_div(Ho, kTH)), 256, Cfg::lds_bytes, st>>>(
-          (const bf16*)x, s.H, s.W, (const bf16*)wpk, b, s.K, act, (bf16*)y, Ho, Wo);
+  kern<<<dim3(ceil_div(Wo, T::TW), ceil_div(Ho, kTH)), 256, Cfg::lds_bytes, st>>>(
+      (const bf16*)x, s.H, s.W, (const bf16*)wpk, b, s.K, act, (bf16*)y, Ho, Wo);
   return true;
 }
 

@@ -102,11 +102,43 @@ __device__ __forceinline__ Vec<LPL> shift_label(const Vec<LPL>& x, int dir, floa
 }
 
 template <int LPL>
+__device__ __forceinline__ Vec<LPL> plus(const Vec<LPL>& x, const Vec<LPL>& y) {
+  Vec<LPL> o;
+#pragma unroll
+  for (int j = 0; j < LPL; ++j) o.v[j] = x.v[j] + y.v[j];
+  return o;
+}
+
+template <int LPL>
 __device__ __forceinline__ float local_min(const Vec<LPL>& x) {
   float m = x.v[0];
 #pragma unroll
   for (int j = 1; j < LPL; ++j) m = fminf(m, x.v[j]);
   return m;
+}
+
+// The same values in new registers.  The sweeps keep their prefetched nodes in
+// a register ring and refill a slot right after reading it; a value read from
+// the slot that stays live past the refill would share its register with the
+// refill load, and the register allocator would then copy the whole ring at the
+// loop back-edge -- a copy that waits for every load in flight.  Copying such
+// values first (the copy waits only for this slot's load, issued a ring length
+// earlier) keeps each slot in fixed registers.
+template <int LPL>
+__device__ __forceinline__ Vec<LPL> fresh(const Vec<LPL>& x) {
+  Vec<LPL> o;
+#pragma unroll
+  for (int j = 0; j < LPL; ++j) asm volatile("v_mov_b32 %0, %1" : "=v"(o.v[j]) : "v"(x.v[j]));
+  return o;
+}
+
+// Makes the values opaque at this point, so that the arithmetic that formed them
+// stays ahead of the slot refill that follows (otherwise it may be sunk past the
+// refill, which again makes the slot's old value outlive it).
+template <int LPL>
+__device__ __forceinline__ void pin(Vec<LPL>& x) {
+#pragma unroll
+  for (int j = 0; j < LPL; ++j) asm volatile("" : "+v"(x.v[j]));
 }
 
 // near(s) = min over 0 < |s - t| < delta of x(t)  (+inf if delta == 1).
@@ -319,12 +351,15 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
 #pragma unroll
   for (int j = 0; j < LPL; ++j) bh.v[j] = 0.f;
   st_node<LPL, FULL>(msg + (long)(n - 1) * D, lane, D, bh);
-  auto step_b = [&](int s, const Vec<LPL>& th) {
-    const int i = n - 2 - s;
-    const float wi = w_lds[i];
+  auto plus_bh = [&](const Vec<LPL>& th) {
     Vec<LPL> U;
 #pragma unroll
     for (int k = 0; k < LPL; ++k) U.v[k] = th.v[k] + bh.v[k];
+    return U;
+  };
+  auto step_b = [&](int s, const Vec<LPL>& U) {
+    const int i = n - 2 - s;
+    const float wi = w_lds[i];
     const float u = wave_min(local_min(U));
     const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, a.L1, a.L2, a.delta);
 #pragma unroll
@@ -340,17 +375,17 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   for (; s0 + P <= nb; s0 += P) {
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      const Vec<LPL> th = theta_of(ra[j], rc[j]);
-      // Keep "read slot j" before "refill slot j": the refill then reuses the
-      // slot's registers and the ring needs no copies at the loop back-edge.
+      // Slot j is consumed (U is a new value) before it is refilled, so the
+      // refill reuses the slot's registers.
+      const Vec<LPL> U = plus_bh(theta_of(ra[j], rc[j]));
       __builtin_amdgcn_sched_barrier(0);
       load_b(s0 + j + P, ra[j], rc[j]);
-      step_b(s0 + j, th);
+      step_b(s0 + j, U);
     }
   }
 #pragma unroll
   for (int j = 0; j < P; ++j)
-    if (s0 + j < nb) step_b(s0 + j, theta_of(ra[j], rc[j]));
+    if (s0 + j < nb) step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])));
   const double bsum = chain_offsets(u_lds, b_lds, n, lane);  // b_0 = sum of all minima
 
   // bh holds Bh[0]; theta[0] is read directly.
@@ -376,12 +411,13 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     Vec<LPL> a_cur = a0;
     // The float64 kept constant (the chain minimum this pass leaves behind) is
     // formed from the per-node minima v_i after the pass: chain_kept().
-    auto step_f = [&](int i, const Vec<LPL>& b, const Vec<LPL>& an, const Vec<LPL>& cn) {
+    // step_f takes RB = R + Bh_i, A_{i+1} and theta_{i+1}.
+    auto step_f = [&](int i, const Vec<LPL>& RB, const Vec<LPL>& an, const Vec<LPL>& thn) {
       const float wi = w_lds[i];
       Vec<LPL> gt, V;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) {
-        gt.v[k] = valid[k] ? kGamma * (R.v[k] + b.v[k]) : 0.f;
+        gt.v[k] = valid[k] ? kGamma * RB.v[k] : 0.f;
         V.v[k] = valid[k] ? R.v[k] - gt.v[k] : kInfF;
       }
       const float v = wave_min(local_min(V));
@@ -392,7 +428,6 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       for (int k = 0; k < LPL; ++k) mm.v[k] = valid[k] ? m.v[k] : -kInfF;
       const Vec<LPL> nmax = near_max<LPL, DC>(mm, a.delta);
       const float wl1 = wi * a.L1;
-      const Vec<LPL> thn = theta_of(an, cn);
       Vec<LPL> out;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) {
@@ -412,17 +447,20 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     for (; i0 + P <= nb; i0 += P) {
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        const Vec<LPL> b = rb[j];
-        const Vec<LPL> an = ra[j];
-        const Vec<LPL> cn = rc[j];
+        // A_{i+1} lives on as the next step's a_cur: copied out of the slot.
+        const Vec<LPL> an = fresh(ra[j]);
+        Vec<LPL> thn = theta_of(an, rc[j]);
+        Vec<LPL> RB = plus(R, rb[j]);
+        pin(thn);
+        pin(RB);
         __builtin_amdgcn_sched_barrier(0);
         load_f(i0 + j + P, rb[j], ra[j], rc[j]);
-        step_f(i0 + j, b, an, cn);
+        step_f(i0 + j, RB, an, thn);
       }
     }
 #pragma unroll
     for (int j = 0; j < P; ++j)
-      if (i0 + j < nb) step_f(i0 + j, rb[j], ra[j], rc[j]);
+      if (i0 + j < nb) step_f(i0 + j, plus(R, rb[j]), ra[j], theta_of(ra[j], rc[j]));
     {
       Vec<LPL> out;
 #pragma unroll
@@ -467,7 +505,9 @@ __device__ __forceinline__ void ring_pass(const float* __restrict__ Ab, long nst
   for (; s0 + P <= cnt; s0 += P) {
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      const Vec<LPL> av = ra[j], mv = rm[j];
+      const Vec<LPL> av = fresh(ra[j]);
+      Vec<LPL> mv;
+      if constexpr (MSG) mv = fresh(rm[j]);
       __builtin_amdgcn_sched_barrier(0);
       load(s0 + j + P, ra[j], rm[j]);
       step(first + dir * (s0 + j), av, mv);
