@@ -245,6 +245,69 @@ __device__ __forceinline__ void st_node(float* __restrict__ base, int lane, int 
   }
 }
 
+// Buffer-resource access for the row sweeps.  A row chain's nodes (and its
+// message buffer) are contiguous, so a node address is the chain's base, held in
+// a buffer resource, plus a 32-bit byte offset in an SGPR: one or two scalar
+// instructions per node instead of 64-bit address arithmetic.  Every offset
+// passed here is that of a node inside the resource's range.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
+
+__device__ __forceinline__ Rsrc chain_rsrc(const float* base, long nfloats) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)(nfloats * 4), 0x00020000);
+}
+
+template <int LPL, bool FULL>
+__device__ __forceinline__ Vec<LPL> ld_buf(Rsrc r, int soff, int lane, int D, float fill) {
+  Vec<LPL> out;
+  if constexpr (FULL && LPL == 4) {
+    const u32x4 t = __builtin_amdgcn_raw_buffer_load_b128(r, lane * 16, soff, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out.v[j] = __uint_as_float(t[j]);
+  } else if constexpr (FULL && LPL == 2) {
+    const u32x2 t = __builtin_amdgcn_raw_buffer_load_b64(r, lane * 8, soff, 0);
+    out.v[0] = __uint_as_float(t[0]);
+    out.v[1] = __uint_as_float(t[1]);
+  } else if constexpr (FULL) {
+    out.v[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, soff, 0));
+  } else {
+#pragma unroll
+    for (int j = 0; j < LPL; ++j) {
+      const int s = lane * LPL + j;
+      const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (s < D ? s : D - 1) * 4, soff, 0));
+      out.v[j] = s < D ? v : fill;
+    }
+  }
+  return out;
+}
+
+template <int LPL, bool FULL>
+__device__ __forceinline__ void st_buf(Rsrc r, int soff, int lane, int D, const Vec<LPL>& x) {
+  if constexpr (FULL && LPL == 4) {
+    const u32x4 t = {__float_as_uint(x.v[0]), __float_as_uint(x.v[1]), __float_as_uint(x.v[2]),
+                     __float_as_uint(x.v[3])};
+    __builtin_amdgcn_raw_buffer_store_b128(t, r, lane * 16, soff, 0);
+  } else if constexpr (FULL && LPL == 2) {
+    const u32x2 t = {__float_as_uint(x.v[0]), __float_as_uint(x.v[1])};
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, lane * 8, soff, 0);
+  } else if constexpr (FULL) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x.v[0]), r, lane * 4, soff, 0);
+  } else {
+#pragma unroll
+    for (int j = 0; j < LPL; ++j) {
+      const int s = lane * LPL + j;
+      if (s < D) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x.v[j]), r, s * 4, soff, 0);
+    }
+  }
+}
+
+// Edge weights of a block of up to 64 steps sit one per lane (lane j: the
+// block's step j), read from LDS a block ahead; a step takes its weight with
+// v_readlane, as an SGPR operand, so the step itself waits on no LDS access.
+__device__ __forceinline__ float lane_weight(float wv, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wv), j));
+}
+
 // Suffix sums b_i = u_i + ... + u_{n-2} (b_{n-1} = 0) of the backward-pass
 // minima, in float64, the 64 lanes each taking a contiguous segment; returns b_0.
 // Done once after the pass so the per-node step carries no float64 work.
@@ -310,6 +373,7 @@ template <int LPL, int MODE, int P, bool FULL, int DC>
 __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
   constexpr bool kCols = MODE == kColsMinorant;
+  static_assert(P <= 64, "one weight per lane and step of a block");
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
   const int n = a.n, D = a.D;
@@ -339,14 +403,26 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   __syncthreads();
 
   Vec<LPL> ra[P], rc[P], rb[P];
+  // Row chains (the host guarantees a row's byte extent fits in 31 bits): the
+  // unaries and the messages through buffer resources; columns use 64-bit
+  // addresses (their nodes span the whole volume).
+  const int strideB = (int)(nstride * 4);
+  const Rsrc rA = chain_rsrc(Ab, (long)(n - 1) * nstride + D);
+  const Rsrc rM = chain_rsrc(msg, (long)n * D);
 
   // ---- backward pass: step s handles i = n-2-s: Bh[i] = minconv(theta[i+1] + Bh[i+1]) - min(...)
   const int nb = n - 1;
   auto load_b = [&](int s, Vec<LPL>& av, Vec<LPL>& cv) {
     const int node = s < nb ? n - 1 - s : 1;  // clamped: beyond the chain re-read node 1
-    av = ld_node<LPL, FULL>(Ab + node * nstride, lane, D, kInfF);
-    if constexpr (kCols) cv = ld_node<LPL, FULL>(Cb + node * nstride, lane, D, kInfF);
+    if constexpr (kCols) {
+      av = ld_node<LPL, FULL>(Ab + node * nstride, lane, D, kInfF);
+      cv = ld_node<LPL, FULL>(Cb + node * nstride, lane, D, kInfF);
+    } else {
+      av = ld_buf<LPL, FULL>(rA, node * strideB, lane, D, kInfF);
+    }
   };
+  // weights of steps s0 .. s0+63 of the backward pass (edge n-2-s)
+  auto wlane_b = [&](int s0) { return w_lds[max(n - 2 - (s0 + lane), 0)]; };
   Vec<LPL> bh;
 #pragma unroll
   for (int j = 0; j < LPL; ++j) bh.v[j] = 0.f;
@@ -357,14 +433,14 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     for (int k = 0; k < LPL; ++k) U.v[k] = th.v[k] + bh.v[k];
     return U;
   };
-  auto step_b = [&](int s, const Vec<LPL>& U) {
+  auto step_b = [&](int s, const Vec<LPL>& U, float wi) {
     const int i = n - 2 - s;
-    const float wi = w_lds[i];
     const float u = wave_min(local_min(U));
     const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, a.L1, a.L2, a.delta);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) bh.v[k] = valid[k] ? m.v[k] - u : 0.f;
-    st_node<LPL, FULL>(msg + (long)i * D, lane, D, bh);
+    if constexpr (kCols) st_node<LPL, FULL>(msg + (long)i * D, lane, D, bh);
+    else st_buf<LPL, FULL>(rM, i * D * 4, lane, D, bh);
     u_lds[i] = u;  // every lane writes the same value
   };
   if (nb > 0) {
@@ -372,7 +448,9 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     for (int j = 0; j < P; ++j) load_b(j, ra[j], rc[j]);
   }
   int s0 = 0;
+  float wv = wlane_b(0);
   for (; s0 + P <= nb; s0 += P) {
+    const float wnext = wlane_b(s0 + P);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       // Slot j is consumed (U is a new value) before it is refilled, so the
@@ -380,12 +458,13 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       const Vec<LPL> U = plus_bh(theta_of(ra[j], rc[j]));
       __builtin_amdgcn_sched_barrier(0);
       load_b(s0 + j + P, ra[j], rc[j]);
-      step_b(s0 + j, U);
+      step_b(s0 + j, U, lane_weight(wv, j));
     }
+    wv = wnext;
   }
 #pragma unroll
   for (int j = 0; j < P; ++j)
-    if (s0 + j < nb) step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])));
+    if (s0 + j < nb) step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])), lane_weight(wv, j));
   const double bsum = chain_offsets(u_lds, b_lds, n, lane);  // b_0 = sum of all minima
 
   // bh holds Bh[0]; theta[0] is read directly.
@@ -403,17 +482,23 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     // Forward pass: keep g_i = gamma (R + Bh) + excess, pass V = R - gamma (R + Bh) on.
     auto load_f = [&](int i, Vec<LPL>& bv, Vec<LPL>& av, Vec<LPL>& cv) {
       const int node = i < nb ? i : nb - 1;
-      bv = ld_node<LPL, FULL>(msg + (long)node * D, lane, D, 0.f);
-      av = ld_node<LPL, FULL>(Ab + (node + 1) * nstride, lane, D, kInfF);
-      if constexpr (kCols) cv = ld_node<LPL, FULL>(Cb + (node + 1) * nstride, lane, D, kInfF);
+      if constexpr (kCols) {
+        bv = ld_node<LPL, FULL>(msg + (long)node * D, lane, D, 0.f);
+        av = ld_node<LPL, FULL>(Ab + (node + 1) * nstride, lane, D, kInfF);
+        cv = ld_node<LPL, FULL>(Cb + (node + 1) * nstride, lane, D, kInfF);
+      } else {
+        bv = ld_buf<LPL, FULL>(rM, node * D * 4, lane, D, 0.f);
+        av = ld_buf<LPL, FULL>(rA, (node + 1) * strideB, lane, D, kInfF);
+      }
     };
+    // weights of steps i0 .. i0+63 of the forward pass (edge i)
+    auto wlane_f = [&](int i0) { return w_lds[max(min(i0 + lane, n - 2), 0)]; };
     Vec<LPL> R = th0;
     Vec<LPL> a_cur = a0;
     // The float64 kept constant (the chain minimum this pass leaves behind) is
     // formed from the per-node minima v_i after the pass: chain_kept().
     // step_f takes RB = R + Bh_i, A_{i+1} and theta_{i+1}.
-    auto step_f = [&](int i, const Vec<LPL>& RB, const Vec<LPL>& an, const Vec<LPL>& thn) {
-      const float wi = w_lds[i];
+    auto step_f = [&](int i, const Vec<LPL>& RB, const Vec<LPL>& an, const Vec<LPL>& thn, float wi) {
       Vec<LPL> gt, V;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) {
@@ -436,7 +521,8 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         out.v[k] = MODE == kRowsMinorant ? a_cur.v[k] - g : a_cur.v[k] + g;
         R.v[k] = valid[k] ? thn.v[k] + m.v[k] - v : kInfF;
       }
-      st_node<LPL, FULL>(Aw + i * nstride, lane, D, out);
+      if constexpr (kCols) st_node<LPL, FULL>(Aw + i * nstride, lane, D, out);
+      else st_buf<LPL, FULL>(rA, i * strideB, lane, D, out);
       a_cur = an;
     };
     if (nb > 0) {
@@ -444,23 +530,28 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       for (int j = 0; j < P; ++j) load_f(j, rb[j], ra[j], rc[j]);
     }
     int i0 = 0;
+    float wf = wlane_f(0);
     for (; i0 + P <= nb; i0 += P) {
+      const float wnext = wlane_f(i0 + P);
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         // A_{i+1} lives on as the next step's a_cur: copied out of the slot.
         const Vec<LPL> an = fresh(ra[j]);
         Vec<LPL> thn = theta_of(an, rc[j]);
         Vec<LPL> RB = plus(R, rb[j]);
-        pin(thn);
-        pin(RB);
+        if constexpr (kCols) {  // (rows: thn is `an` itself)
+          pin(thn);
+          pin(RB);
+        }
         __builtin_amdgcn_sched_barrier(0);
         load_f(i0 + j + P, rb[j], ra[j], rc[j]);
-        step_f(i0 + j, RB, an, thn);
+        step_f(i0 + j, RB, an, thn, lane_weight(wf, j));
       }
+      wf = wnext;
     }
 #pragma unroll
     for (int j = 0; j < P; ++j)
-      if (i0 + j < nb) step_f(i0 + j, plus(R, rb[j]), ra[j], theta_of(ra[j], rc[j]));
+      if (i0 + j < nb) step_f(i0 + j, plus(R, rb[j]), ra[j], theta_of(ra[j], rc[j]), lane_weight(wf, j));
     {
       Vec<LPL> out;
 #pragma unroll
@@ -486,23 +577,28 @@ __device__ double chain_total(const float* u, int n, int lane) {
   return wave_sum_f64(tot);
 }
 
-// Runs step(node, A_node+aoff, msg_node) for cnt nodes first, first + dir, ...,
-// with the unaries (and, with MSG, the message buffer) P nodes ahead in a
-// register ring, as in k_chain.
+// Runs step(i, A_{i+aoff}, msg_i, w_{i+woff}) for the cnt nodes i = first,
+// first + dir, ..., with the unaries (and, with MSG, the message buffer) P nodes
+// ahead in a register ring, as in k_chain, and the edge weights w (in LDS, n - 1
+// of them) one block ahead in lanes.
 template <int LPL, bool FULL, int P, bool MSG, typename Step>
-__device__ __forceinline__ void ring_pass(const float* __restrict__ Ab, long nstride, const float* msg, int D,
-                                          int lane, int first, int cnt, int dir, int aoff, Step&& step) {
+__device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const float* w_lds, int n, int D,
+                                          int lane, int first, int cnt, int dir, int aoff, int woff, Step&& step) {
+  static_assert(P <= 64, "one weight per lane and step of a block");
   if (cnt <= 0) return;
   Vec<LPL> ra[P], rm[P];
   auto load = [&](int s, Vec<LPL>& av, Vec<LPL>& mv) {
     const int i = first + dir * (s < cnt ? s : cnt - 1);
-    av = ld_node<LPL, FULL>(Ab + (i + aoff) * nstride, lane, D, kInfF);
-    if constexpr (MSG) mv = ld_node<LPL, FULL>(msg + (long)i * D, lane, D, 0.f);
+    av = ld_buf<LPL, FULL>(rA, (i + aoff) * strideB, lane, D, kInfF);
+    if constexpr (MSG) mv = ld_buf<LPL, FULL>(rM, i * D * 4, lane, D, 0.f);
   };
+  auto wlane = [&](int s0) { return w_lds[max(min(first + dir * (s0 + lane) + woff, n - 2), 0)]; };
 #pragma unroll
   for (int j = 0; j < P; ++j) load(j, ra[j], rm[j]);
   int s0 = 0;
+  float wv = wlane(0);
   for (; s0 + P <= cnt; s0 += P) {
+    const float wnext = wlane(s0 + P);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const Vec<LPL> av = fresh(ra[j]);
@@ -510,12 +606,13 @@ __device__ __forceinline__ void ring_pass(const float* __restrict__ Ab, long nst
       if constexpr (MSG) mv = fresh(rm[j]);
       __builtin_amdgcn_sched_barrier(0);
       load(s0 + j + P, ra[j], rm[j]);
-      step(first + dir * (s0 + j), av, mv);
+      step(first + dir * (s0 + j), av, mv, lane_weight(wv, j));
     }
+    wv = wnext;
   }
 #pragma unroll
   for (int j = 0; j < P; ++j)
-    if (s0 + j < cnt) step(first + dir * (s0 + j), ra[j], rm[j]);
+    if (s0 + j < cnt) step(first + dir * (s0 + j), ra[j], rm[j], lane_weight(wv, j));
 }
 
 // Row decode with both message passes at once: wave 0 runs the forward
@@ -539,6 +636,9 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   float* msg = a.msg + (long)c * n * D;
   float* w_lds = reinterpret_cast<float*>(lds2);
   float* u_lds = w_lds + n;
+  const int strideB = (int)(nstride * 4);
+  const Rsrc rA = chain_rsrc(Ab, (long)(n - 1) * nstride + D);
+  const Rsrc rM = chain_rsrc(msg, (long)n * D);
   bool valid[LPL];
 #pragma unroll
   for (int j = 0; j < LPL; ++j) valid[j] = FULL || lane * LPL + j < D;
@@ -547,18 +647,18 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   const int mid = n / 2;
 
   Vec<LPL> X;  // wave 0: F_i, wave 1: Bh_i
-  auto fwd = [&](int i, const Vec<LPL>& an) {  // F_i = A_i + minconv(F_{i-1}) - min F_{i-1}
+  auto fwd = [&](const Vec<LPL>& an, float w) {  // F_i = A_i + minconv(F_{i-1}; w_{i-1}) - min F_{i-1}
     const float f = wave_min(local_min(X));
-    const Vec<LPL> m = minconv<LPL, DC>(X, f, w_lds[i - 1], a.L1, a.L2, a.delta);
+    const Vec<LPL> m = minconv<LPL, DC>(X, f, w, a.L1, a.L2, a.delta);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
   };
-  auto bwd = [&](int i, const Vec<LPL>& an1) {  // Bh_i from Bh_{i+1} and A_{i+1}
+  auto bwd = [&](int i, const Vec<LPL>& an1, float w) {  // Bh_i from Bh_{i+1}, A_{i+1} and w_i
     Vec<LPL> U;
 #pragma unroll
     for (int k = 0; k < LPL; ++k) U.v[k] = (valid[k] ? an1.v[k] : kInfF) + X.v[k];
     const float u = wave_min(local_min(U));
-    const Vec<LPL> m = minconv<LPL, DC>(U, u, w_lds[i], a.L1, a.L2, a.delta);
+    const Vec<LPL> m = minconv<LPL, DC>(U, u, w, a.L1, a.L2, a.delta);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? m.v[k] - u : 0.f;
     u_lds[i] = u;  // every lane writes the same value
@@ -594,34 +694,34 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   if (wv == 0) {
     X = ld_node<LPL, FULL>(Ab, lane, D, kInfF);  // F_0 = A_0
     if (mid > 0) st_node<LPL, FULL>(msg, lane, D, X);
-    ring_pass<LPL, FULL, P, false>(Ab, nstride, msg, D, lane, 1, mid - 1, 1, 0,
-                                   [&](int i, const Vec<LPL>& an, const Vec<LPL>&) {
-                                     fwd(i, an);
-                                     st_node<LPL, FULL>(msg + (long)i * D, lane, D, X);
+    ring_pass<LPL, FULL, P, false>(rA, strideB, rM, w_lds, n, D, lane, 1, mid - 1, 1, 0, -1,
+                                   [&](int i, const Vec<LPL>& an, const Vec<LPL>&, float w) {
+                                     fwd(an, w);
+                                     st_buf<LPL, FULL>(rM, i * D * 4, lane, D, X);
                                    });
   } else {
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = 0.f;  // Bh_{n-1} = 0
     st_node<LPL, FULL>(msg + (long)(n - 1) * D, lane, D, X);
-    ring_pass<LPL, FULL, P, false>(Ab, nstride, msg, D, lane, n - 2, n - 1 - mid, -1, 1,
-                                   [&](int i, const Vec<LPL>& an1, const Vec<LPL>&) {
-                                     bwd(i, an1);
-                                     st_node<LPL, FULL>(msg + (long)i * D, lane, D, X);
+    ring_pass<LPL, FULL, P, false>(rA, strideB, rM, w_lds, n, D, lane, n - 2, n - 1 - mid, -1, 1, 0,
+                                   [&](int i, const Vec<LPL>& an1, const Vec<LPL>&, float w) {
+                                     bwd(i, an1, w);
+                                     st_buf<LPL, FULL>(rM, i * D * 4, lane, D, X);
                                    });
   }
   __threadfence_block();
   __syncthreads();  // executed by both waves: the other half's messages are now visible
   // Phase 2: each wave continues into the other half and decodes it.
   if (wv == 0) {
-    ring_pass<LPL, FULL, P, true>(Ab, nstride, msg, D, lane, mid, n - mid, 1, 0,
-                                  [&](int i, const Vec<LPL>& an, const Vec<LPL>& bh) {
-                                    if (i > 0) fwd(i, an);
+    ring_pass<LPL, FULL, P, true>(rA, strideB, rM, w_lds, n, D, lane, mid, n - mid, 1, 0, -1,
+                                  [&](int i, const Vec<LPL>& an, const Vec<LPL>& bh, float w) {
+                                    if (i > 0) fwd(an, w);
                                     decode(i, X, bh);
                                   });
   } else {
-    ring_pass<LPL, FULL, P, true>(Ab, nstride, msg, D, lane, mid - 1, mid, -1, 1,
-                                  [&](int i, const Vec<LPL>& an1, const Vec<LPL>& F) {
-                                    bwd(i, an1);
+    ring_pass<LPL, FULL, P, true>(rA, strideB, rM, w_lds, n, D, lane, mid - 1, mid, -1, 1, 0,
+                                  [&](int i, const Vec<LPL>& an1, const Vec<LPL>& F, float w) {
+                                    bwd(i, an1, w);
                                     decode(i, F, X);
                                   });
     // Chain minimum: sum of the backward minima + min(A_0 + Bh_0).
@@ -680,6 +780,12 @@ static int launch_mode(ChainArgs& a, hipStream_t st) {
 }
 
 int launch_chain(ChainMode mode, ChainArgs& a, hipStream_t st) {
+  if (mode != kColsMinorant) {  // row chains are addressed with 32-bit byte offsets
+    const long extent = ((long)(a.n - 1) * a.pns * a.D + a.D) * (long)sizeof(float);
+    const long msg = (long)a.n * a.D * (long)sizeof(float);
+    if (extent > 0x7fffffffL || msg > 0x7fffffffL)
+      return fail(CCS_ERR_ARG, "row chain of %d nodes x %d labels exceeds 2 GiB", a.n, a.D);
+  }
   switch (mode) {
     case kRowsMinorant: return launch_mode<kRowsMinorant>(a, st);
     case kColsMinorant: return launch_mode<kColsMinorant>(a, st);
