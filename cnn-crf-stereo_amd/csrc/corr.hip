@@ -9,9 +9,9 @@
 // feature rows are staged in LDS (f1 as the window of columns the band needs,
 // zero outside the image), each wave computes the 16 x (16*NTM) block
 // S[w][w'] of its 16 pixels against the window with MFMA (a banded GEMM over
-// the channels), parks it in LDS, and the epilogue gathers the band
-// w' = w + o_k, applies -softmax over k and writes the H x W x D volume with
-// 16-byte stores, D contiguous floats per pixel.
+// the channels), parks it in LDS over the dead feature tiles, and the epilogue
+// gathers the band w' = w + o_k, applies -softmax over k and writes the
+// H x W x D volume with 16-byte stores, D contiguous floats per pixel.
 #include <cmath>
 #include <vector>
 
@@ -41,10 +41,11 @@ struct CorrArgs {
   int pitch;      // LDS pixel pitch (elements)
   int win;        // f1 window columns
   int sp;         // S row pitch (floats)
+  int stats_off;  // byte offset of the softmax statistics in LDS
 };
 
 template <typename T, int NTM>
-__global__ __launch_bounds__(256) void k_corr_volume(CorrArgs a) {
+__global__ __launch_bounds__(512) void k_corr_volume(CorrArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int NW = blockDim.x >> 6;
   const int TW = 16 * NW;
@@ -55,9 +56,10 @@ __global__ __launch_bounds__(256) void k_corr_volume(CorrArgs a) {
   const int Cs = a.Cs, P = a.pitch;
   T* F0 = reinterpret_cast<T*>(lds);
   T* F1 = F0 + TW * P;
-  float* S = reinterpret_cast<float*>(F1 + a.win * P);
+  // S takes the feature tiles' place once every wave has finished its MFMAs.
+  float* S = reinterpret_cast<float*>(lds);
   float* S_w = S + wave * 16 * a.sp;
-  float* stats = S + NW * 16 * a.sp;  // [NW][16][2]
+  float* stats = reinterpret_cast<float*>(lds + a.stats_off);  // [NW][16]
 
   const T* f0 = reinterpret_cast<const T*>(a.f0) + (long)h * a.W * Cs;
   const T* f1 = reinterpret_cast<const T*>(a.f1) + (long)h * a.W * Cs;
@@ -123,6 +125,7 @@ __global__ __launch_bounds__(256) void k_corr_volume(CorrArgs a) {
       }
     }
   }
+  __syncthreads();  // the feature tiles are dead: S overwrites them
   // S_w[r][j] = f0[pixel 16*wave + r] . f1[window column 16*wave + j]
 #pragma unroll
   for (int t = 0; t < NTM; ++t)
@@ -302,10 +305,17 @@ __global__ void k_sum_f64_corr(const double* __restrict__ in, int n, double* __r
   }
 }
 
+// LDS bytes of one workgroup: the feature tiles, later overlaid by S, then the statistics.
+static size_t corr_lds_bytes(CorrArgs& a, int NW, size_t esz) {
+  const size_t feat = (size_t)(16 * NW + a.win) * a.pitch * esz;
+  const size_t sbytes = (size_t)NW * 16 * a.sp * 4;
+  a.stats_off = (int)((feat > sbytes ? feat : sbytes) + 15) / 16 * 16;
+  return (size_t)a.stats_off + (size_t)NW * 16 * 4;
+}
+
 template <typename T, int NTM>
 static int launch_volume(CorrArgs a, int NW, hipStream_t st) {
-  const size_t bytes = (size_t)(16 * NW + a.win) * a.pitch * sizeof(T) + (size_t)NW * 16 * a.sp * 4 +
-                       (size_t)NW * 16 * 2 * 4;
+  const size_t bytes = corr_lds_bytes(a, NW, sizeof(T));
   static bool configured = false;
   if (!configured) {
     hipFuncSetAttribute((const void*)k_corr_volume<T, NTM>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -345,13 +355,17 @@ extern "C" int ccs_correlation_volume(ccs_handle_t h, const void* f0, const void
   a.sp = 16 * ntm;
   const size_t esz = precision == CCS_BF16 ? 2 : 4;
   a.pitch = precision == CCS_BF16 ? a.Cs : a.Cs + 1;
-  int NW = 4;
-  for (; NW >= 1; NW >>= 1) {
-    a.win = 16 * (NW - 1) + 16 * ntm;
-    const size_t bytes = (size_t)(16 * NW + a.win) * a.pitch * esz + (size_t)NW * 16 * a.sp * 4 + NW * 16 * 8;
-    if (bytes <= 160 * 1024) break;
+  // Waves per workgroup: the most resident waves per CU (160 KB of LDS), ties
+  // to the wider tile (its f1 window is shared by more pixels).
+  int NW = 0, best = 0;
+  for (int nw = 8; nw >= 1; nw >>= 1) {
+    a.win = 16 * (nw - 1) + 16 * ntm;
+    const size_t bytes = corr_lds_bytes(a, nw, esz);
+    const int resident = nw * (int)((160 * 1024) / bytes);
+    if (resident > best) best = resident, NW = nw;
   }
   CCS_CHECK(NW >= 1, CCS_ERR_UNSUPPORTED, "ccs_correlation_volume: tile does not fit in LDS");
+  a.win = 16 * (NW - 1) + 16 * ntm;
   hipStream_t st = h->stream;
   if (precision == CCS_BF16) {
     switch (ntm) {
