@@ -76,6 +76,27 @@ typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
 __device__ __forceinline__ float bf16_to_f32(bf16 v) { return (float)v; }
 __device__ __forceinline__ bf16 f32_to_bf16(float v) { return (bf16)v; }
 
+// tanh in float32 without branches: for |x| < 0.625 the odd polynomial
+// x + x^3 q(x^2), q of degree 4 fitted (near-minimax, relative error) to
+// (tanh(x) - x) / x^3 on that range, 0.8 ulp in float32 arithmetic; above it
+// 1 - 2e / (1 + e) with e = exp(-2|x|) from v_exp_f32 and v_rcp_f32 (1.5 ulp
+// with exact exp2 and reciprocal; each hardware approximation adds about one).
+// Both branches are evaluated and selected, so a wave never diverges; libm's
+// tanhf branches on |x| and runs both sides for mixed waves.
+__device__ __forceinline__ float tanh_fast(float x) {
+  const float ax = fabsf(x);
+  const float s = x * x;
+  float q = -0.005717044696211815f;
+  q = fmaf(q, s, 0.020650586113333702f);
+  q = fmaf(q, s, -0.05374349653720856f);
+  q = fmaf(q, s, 0.13331490755081177f);
+  q = fmaf(q, s, -0.3333328366279602f);
+  const float p = fmaf(ax * s, q, ax);
+  const float e = __builtin_amdgcn_exp2f(ax * -2.8853900817779268f);  // exp(-2|x|)
+  const float t = fmaf(-2.f * e, __builtin_amdgcn_rcpf(1.f + e), 1.f);
+  return copysignf(ax < 0.625f ? p : t, x);
+}
+
 // ---------------------------------------------------------------------------
 // DPP helpers (wave64).  Controls: quad_perm 0xb1 = [1,0,3,2], 0x4e = [2,3,0,1],
 // row_shr:n = 0x110+n, row_bcast:15 = 0x142, row_bcast:31 = 0x143,

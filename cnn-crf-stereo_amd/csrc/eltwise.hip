@@ -36,7 +36,7 @@ __global__ void k_bias_nchw(float* __restrict__ y, long HW, int C, const float* 
 // y = tanh(y) -- stereo/operators/activation.cpp:33-40 (CUDNN_ACTIVATION_TANH).
 __global__ void k_tanh(float* __restrict__ y, long n) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    y[i] = tanhf(y[i]);
+    y[i] = tanh_fast(y[i]);
 }
 
 // Softmax over the D channels of each pixel of a channel-last volume, one wave

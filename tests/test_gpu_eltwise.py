@@ -33,6 +33,30 @@ def test_tanh(handle):
     torch.testing.assert_close(x, want, rtol=2e-6, atol=1e-7)
 
 
+def test_tanh_ulp_and_special_values(handle):
+    """The branch-free tanh: within 4 ulp of the float64 tanh of the float32
+    input everywhere (dense around the 0.625 switch point, tiny and large
+    arguments), odd, and exact at 0, -0 and +-inf."""
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([np.linspace(-10, 10, 200001), np.linspace(0.6, 0.65, 20001),
+                         np.exp(rng.uniform(np.log(1e-30), np.log(50.0), 100000)),
+                         [0.0, 0.625, np.nextafter(np.float32(0.625), np.float32(0)), 1e-38, 88.0, 1e4]])
+    xs = np.concatenate([xs, -xs]).astype(np.float32)
+    y = torch.as_tensor(xs, device="cuda").clone()
+    call("ccs_tanh_forward", handle.h, ptr(y), y.numel())
+    got = y.cpu().numpy()
+    ref = np.tanh(xs.astype(np.float64))
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    ulp = np.maximum(ulp, np.float64(np.finfo(np.float32).smallest_subnormal))
+    assert np.max(np.abs(got - ref) / ulp) <= 4.0
+    np.testing.assert_array_equal(got[: len(got) // 2], -got[len(got) // 2:])
+    sp = torch.tensor([0.0, -0.0, np.inf, -np.inf], device="cuda")
+    call("ccs_tanh_forward", handle.h, ptr(sp), 4)
+    out = sp.cpu().numpy()
+    np.testing.assert_array_equal(out, [0.0, -0.0, 1.0, -1.0])
+    assert np.signbit(out[1]) and not np.signbit(out[0])
+
+
 @pytest.mark.parametrize("D,negate", [(128, 1), (37, 0), (256, 1), (1, 1)])
 def test_softmax_channels(handle, D, negate):
     g = torch.Generator(device="cuda").manual_seed(D)
