@@ -622,7 +622,7 @@ __device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const f
 // sit in the buffer (wave 0 the right half, wave 1 the left half).  F_i + Bh_i
 // and the decision rule are those of the single-wave decode, so the labels are
 // the same; the chain latency is n steps instead of 2n.
-template <int LPL, int P, bool FULL, int DC>
+template <int LPL, int P, bool FULL, int DC, bool GAP>
 __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   extern __shared__ double lds2[];
   const int c = blockIdx.x;
@@ -663,7 +663,11 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? m.v[k] - u : 0.f;
     u_lds[i] = u;  // every lane writes the same value
   };
-  auto decode = [&](int i, const Vec<LPL>& F, const Vec<LPL>& b) {
+  // Label (and, with GAP, the min-marginal gap) of node i from F_i + Bh_i.
+  // Every lane stores the same value to the same address (one write, no
+  // branch), so a decision has no control flow and its instructions can
+  // interleave with the next node's message update.
+  auto decide = [&](int i, const Vec<LPL>& F, const Vec<LPL>& b) {
     const long p = pbase + (long)i * a.pns;
     float best = kInfF;
 #pragma unroll
@@ -677,17 +681,16 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
     const unsigned long long ball = __ballot(bj < LPL);
     const int first = ball ? __builtin_ctzll(ball) : 0;  // empty only for NaN costs
     const int label = __builtin_amdgcn_readlane(lane * LPL + (bj < LPL ? bj : 0), first);
-    if (a.gap) {
+    if constexpr (GAP) {
       float second = kInfF;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) {
         const float mm = valid[k] ? F.v[k] + b.v[k] : kInfF;
         if (lane * LPL + k != label) second = fminf(second, mm);
       }
-      const float gsecond = wave_min(second);
-      if (lane == 0) a.gap[p] = gsecond - gbest;
+      a.gap[p] = wave_min(second) - gbest;
     }
-    if (lane == 0) a.labels[p] = label;
+    a.labels[p] = label;
   };
 
   // Phase 1: each wave walks to the midpoint, storing its messages.
@@ -711,19 +714,37 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   }
   __threadfence_block();
   __syncthreads();  // executed by both waves: the other half's messages are now visible
-  // Phase 2: each wave continues into the other half and decodes it.
-  if (wv == 0) {
-    ring_pass<LPL, FULL, P, true>(rA, strideB, rM, w_lds, n, D, lane, mid, n - mid, 1, 0, -1,
+  // Phase 2: each wave continues into the other half and decodes it.  The
+  // decision for a node is taken one step later, beside the next node's
+  // message update (two independent dependency chains in one step); the
+  // first node of the half is handled before the ring, the last after it.
+  Vec<LPL> Fp, Bp;  // min-marginal terms of the node whose decision is pending
+  if (wv == 0) {     // nodes mid .. n-1, forward messages F
+    if (mid > 0) fwd(ld_buf<LPL, FULL>(rA, mid * strideB, lane, D, kInfF), w_lds[mid - 1]);
+    Fp = X;
+    Bp = ld_buf<LPL, FULL>(rM, mid * D * 4, lane, D, 0.f);
+    ring_pass<LPL, FULL, P, true>(rA, strideB, rM, w_lds, n, D, lane, mid + 1, n - mid - 1, 1, 0, -1,
                                   [&](int i, const Vec<LPL>& an, const Vec<LPL>& bh, float w) {
-                                    if (i > 0) fwd(an, w);
-                                    decode(i, X, bh);
+                                    decide(i - 1, Fp, Bp);
+                                    fwd(an, w);
+                                    Fp = X;
+                                    Bp = bh;
                                   });
-  } else {
-    ring_pass<LPL, FULL, P, true>(rA, strideB, rM, w_lds, n, D, lane, mid - 1, mid, -1, 1, 0,
+    decide(n - 1, Fp, Bp);
+  } else if (mid > 0) {  // nodes mid-1 .. 0, backward messages Bh
+    bwd(mid - 1, ld_buf<LPL, FULL>(rA, mid * strideB, lane, D, kInfF), w_lds[mid - 1]);
+    Fp = ld_buf<LPL, FULL>(rM, (mid - 1) * D * 4, lane, D, 0.f);
+    Bp = X;
+    ring_pass<LPL, FULL, P, true>(rA, strideB, rM, w_lds, n, D, lane, mid - 2, mid - 1, -1, 1, 0,
                                   [&](int i, const Vec<LPL>& an1, const Vec<LPL>& F, float w) {
+                                    decide(i + 1, Fp, Bp);
                                     bwd(i, an1, w);
-                                    decode(i, F, X);
+                                    Fp = F;
+                                    Bp = X;
                                   });
+    decide(0, Fp, Bp);
+  }
+  if (wv == 1) {
     // Chain minimum: sum of the backward minima + min(A_0 + Bh_0).
     const Vec<LPL> a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
     Vec<LPL> M0;
@@ -742,7 +763,10 @@ static void launch_one(ChainArgs& a, size_t lds, hipStream_t st) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if constexpr (MODE == kRowsDecode) {
     const size_t lds2 = 2 * sizeof(float) * (size_t)a.n;
-    k_decode2<LPL, P, FULL, DC><<<a.n_chains, 128, lds2, st>>>(a);
+    if (a.gap)
+      k_decode2<LPL, P, FULL, DC, true><<<a.n_chains, 128, lds2, st>>>(a);
+    else
+      k_decode2<LPL, P, FULL, DC, false><<<a.n_chains, 128, lds2, st>>>(a);
   } else {
     k_chain<LPL, MODE, P, FULL, DC><<<a.n_chains, 64, lds, st>>>(a);
   }
