@@ -42,6 +42,11 @@ constexpr float kInfF = __builtin_huge_valf();
 #endif
 constexpr float kTieEps = CCS_CRF_TIE_EPS;
 
+// Scheduling fence between a ring slot's read and its refill: memory, LDS and
+// scalar instructions keep their side of it, VALU instructions may cross (they
+// fill the DPP/permlane wait states of the neighbouring step).
+constexpr int kSchedValu = 0x2;
+
 template <int LPL>
 struct Vec {
   float v[LPL];
@@ -456,7 +461,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       // Slot j is consumed (U is a new value) before it is refilled, so the
       // refill reuses the slot's registers.
       const Vec<LPL> U = plus_bh(theta_of(ra[j], rc[j]));
-      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_sched_barrier(kSchedValu);
       load_b(s0 + j + P, ra[j], rc[j]);
       step_b(s0 + j, U, lane_weight(wv, j));
     }
@@ -543,7 +548,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
           pin(thn);
           pin(RB);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(kSchedValu);
         load_f(i0 + j + P, rb[j], ra[j], rc[j]);
         step_f(i0 + j, RB, an, thn, lane_weight(wf, j));
       }
@@ -604,7 +609,7 @@ __device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const f
       const Vec<LPL> av = fresh(ra[j]);
       Vec<LPL> mv;
       if constexpr (MSG) mv = fresh(rm[j]);
-      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_sched_barrier(kSchedValu);
       load(s0 + j + P, ra[j], rm[j]);
       step(first + dir * (s0 + j), av, mv, lane_weight(wv, j));
     }
