@@ -510,7 +510,7 @@ static bool try_bf16(const ConvShape& s, const void* x, const void* wpk, const f
   const int Ho = s.H - KH + 1, Wo = s.W - KW + 1;
   static bool configured = false;
   if (!configured) {
-    hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
     configured = true;
   }
   kern<<<dim3(ceil_div(Wo, T::TW), ceil_div(Ho, kTH)), 256, Cfg::lds_bytes, st>>>(
@@ -527,7 +527,7 @@ static bool try_f32(const ConvShape& s, const void* x, const void* wpk, const fl
   const size_t lds = Cfg::a_bytes + (size_t)Cfg::QG * Cfg::chunk_bytes;
   static bool configured = false;
   if (!configured) {
-    hipFuncSetAttribute((const void*)k_conv_f32<KH, KW, CG, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void*)k_conv_f32<KH, KW, CG, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                         kLdsBytes);
     configured = true;
   }

@@ -318,7 +318,8 @@ static int launch_volume(CorrArgs a, int NW, hipStream_t st) {
   const size_t bytes = corr_lds_bytes(a, NW, sizeof(T));
   static bool configured = false;
   if (!configured) {
-    hipFuncSetAttribute((const void*)k_corr_volume<T, NTM>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    CCS_HIP(hipFuncSetAttribute((const void*)k_corr_volume<T, NTM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024));
     configured = true;
   }
   k_corr_volume<T, NTM><<<dim3(ceil_div(a.W, 16 * NW), a.H), 64 * NW, bytes, st>>>(a);

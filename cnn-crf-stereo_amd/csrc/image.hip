@@ -15,25 +15,27 @@ __device__ __forceinline__ int reflect(int i, int n) {
   return i < n ? i : period - 1 - i;
 }
 
-// out[c][y][x] = img[c][reflect(y-p)][reflect(x-p)]  (stereo/stereo.cu:879-889).
-__global__ void k_pad(const float* __restrict__ img, int H, int W, int p, float* __restrict__ out) {
-  const int Wo = W + 2 * p, Ho = H + 2 * p;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
-  const int c = blockIdx.z;
-  if (x >= Wo) return;
-  out[((long)c * Ho + y) * Wo + x] =
-      img[((long)c * H + reflect(y - p, H)) * W + reflect(x - p, W)];
-}
+constexpr int kMomentBlocks = 128;
 
-// Per-block partial sums of x and x^2 in float64; grid (nb, C).
-__global__ void k_moments_partial(const float* __restrict__ x, long n, double* __restrict__ part) {
-  const float* xc = x + (long)blockIdx.y * n;
+// Pass 1: out[c][y][x] = img[c][reflect(y-p)][reflect(x-p)] (stereo/stereo.cu:879-889)
+// and per-block partial sums of the padded values and their squares in float64.
+// Grid (kMomentBlocks, C), 256 threads; block b takes padded rows b, b + 128, ...
+__global__ __launch_bounds__(256) void k_pad_moments(const float* __restrict__ img, int H, int W, int p,
+                                                     float* __restrict__ out, double* __restrict__ part) {
+  const int Wo = W + 2 * p, Ho = H + 2 * p;
+  const int c = blockIdx.y;
+  const float* ic = img + (long)c * H * W;
+  float* oc = out + (long)c * Ho * Wo;
   double s = 0.0, q = 0.0;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const double v = xc[i];
-    s += v;
-    q += v * v;
+  for (int y = blockIdx.x; y < Ho; y += gridDim.x) {
+    const float* src = ic + (long)reflect(y - p, H) * W;
+    float* dst = oc + (long)y * Wo;
+    for (int x = threadIdx.x; x < Wo; x += blockDim.x) {
+      const float v = src[reflect(x - p, W)];
+      dst[x] = v;
+      s += (double)v;
+      q += (double)v * (double)v;
+    }
   }
   __shared__ double ss[256], sq[256];
   ss[threadIdx.x] = s;
@@ -47,35 +49,43 @@ __global__ void k_moments_partial(const float* __restrict__ x, long n, double* _
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[2 * ((long)blockIdx.y * gridDim.x + blockIdx.x)] = ss[0];
-    part[2 * ((long)blockIdx.y * gridDim.x + blockIdx.x) + 1] = sq[0];
+    part[2 * ((long)c * gridDim.x + blockIdx.x)] = ss[0];
+    part[2 * ((long)c * gridDim.x + blockIdx.x) + 1] = sq[0];
   }
 }
 
-// mean = sum / N, std = sqrt(sum x^2 / N - mean^2) (population, stereo/utils.cu:160-237).
-__global__ void k_moments_final(const double* __restrict__ part, int nb, long n, double* __restrict__ stats,
-                                int C) {
-  const int c = threadIdx.x;
-  if (c >= C) return;
-  double s = 0.0, q = 0.0;
-  for (int b = 0; b < nb; ++b) {
-    s += part[2 * ((long)c * nb + b)];
-    q += part[2 * ((long)c * nb + b) + 1];
-  }
-  const double mean = s / (double)n;
-  const double var = q / (double)n - mean * mean;
-  stats[c] = mean;
-  stats[C + c] = var > 0.0 ? sqrt(var) : 0.0;
-}
-
-__global__ void k_normalize(float* __restrict__ x, long n, const double* __restrict__ stats, int C) {
+// Pass 2: every block sums the kMomentBlocks partials of its channel in the same
+// order (so all blocks agree bit for bit), forms mean = sum / N and the
+// population std = sqrt(sum x^2 / N - mean^2) (stereo/utils.cu:160-237), block 0
+// publishes them, and the block normalises its share: x = (x - mean) / std.
+__global__ __launch_bounds__(256) void k_normalize(float* __restrict__ x, long n, const double* __restrict__ part,
+                                                   double* __restrict__ stats, int C) {
   const int c = blockIdx.y;
-  const double mean = stats[c];
-  const double sd = stats[C + c];
+  __shared__ double ss[kMomentBlocks], sq[kMomentBlocks];
+  if (threadIdx.x < kMomentBlocks) {
+    ss[threadIdx.x] = part[2 * ((long)c * kMomentBlocks + threadIdx.x)];
+    sq[threadIdx.x] = part[2 * ((long)c * kMomentBlocks + threadIdx.x) + 1];
+  }
+  __syncthreads();
+  for (int o = kMomentBlocks / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      ss[threadIdx.x] += ss[threadIdx.x + o];
+      sq[threadIdx.x] += sq[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  const double mean = ss[0] / (double)n;
+  const double var = sq[0] / (double)n - mean * mean;
+  const double sd = var > 0.0 ? sqrt(var) : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    stats[c] = mean;
+    stats[C + c] = sd;
+  }
   const float inv = sd > 0.0 ? (float)(1.0 / sd) : 1.0f;
   const float m = (float)mean;
+  float* xc = x + (long)c * n;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    x[(long)c * n + i] = (x[(long)c * n + i] - m) * inv;
+    xc[i] = (xc[i] - m) * inv;
 }
 
 __global__ void k_rgb2gray(const float* __restrict__ rgb, long n, float* __restrict__ gray) {
@@ -169,15 +179,13 @@ extern "C" int ccs_pad_normalize(ccs_handle_t h, const float* img, int C, int H,
             "ccs_pad_normalize: bad shape C=%d H=%d W=%d pad=%d", C, H, W, pad);
   const int Ho = H + 2 * pad, Wo = W + 2 * pad;
   const long n = (long)Ho * Wo;
-  const int nb = 128;
+  const int nb = kMomentBlocks;
   int rc = h->misc.reserve(sizeof(double) * (2 * (size_t)C * nb + 2 * C));
   if (rc) return rc;
   double* part = (double*)h->misc.ptr;
   double* st = stats ? stats : part + 2 * (size_t)C * nb;
-  k_pad<<<dim3(ceil_div(Wo, 256), Ho, C), 256, 0, h->stream>>>(img, H, W, pad, out);
-  k_moments_partial<<<dim3(nb, C), 256, 0, h->stream>>>(out, n, part);
-  k_moments_final<<<1, 64, 0, h->stream>>>(part, nb, n, st, C);
-  k_normalize<<<dim3(256, C), 256, 0, h->stream>>>(out, n, st, C);
+  k_pad_moments<<<dim3(nb, C), 256, 0, h->stream>>>(img, H, W, pad, out, part);
+  k_normalize<<<dim3(256, C), 256, 0, h->stream>>>(out, n, part, st, C);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }
