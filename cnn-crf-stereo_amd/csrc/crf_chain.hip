@@ -400,8 +400,9 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     return th;
   };
 
-  // The chain's edge weights go to LDS once; each step reads its weight there.
-  // u_lds takes the per-node minima of a pass (float64 sums are formed after it).
+  // The chain's edge weights go to LDS once (the sweeps read them a block of
+  // steps at a time, one per lane).  u_lds takes the per-node minima of a pass
+  // (float64 sums are formed after it).
   float* w_lds = reinterpret_cast<float*>(b_lds + n);
   float* u_lds = w_lds + n;
   for (int i = lane; i < n - 1; i += 64) w_lds[i] = wb[(long)i * a.pns];
