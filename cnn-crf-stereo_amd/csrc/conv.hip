@@ -537,22 +537,27 @@ static bool try_f32(const ConvShape& s, const void* x, const void* wpk, const fl
 }
 
 // MFMA kernels exist for the 100-channel layers of both tower architectures
-// (3x3 and 2x2) and for 8-channel layers (small test networks).  They write the
-// output with the input's channel stride, so both strides must agree.
+// (3x3 and 2x2), the 64-channel 3x3 layer of the pairwise network
+// (stereo/stereonet.cpp:158-164) and 8-channel layers (small test networks).
+// They write the output with the input's channel stride, so both strides must
+// agree.
 static bool mfma_supported(const ConvShape& s) {
   const int cs = feature_stride(s.C), nt = (s.K + 15) / 16;
-  const bool kern = (s.KH == 3 && s.KW == 3) || (s.KH == 2 && s.KW == 2);
-  return kern && feature_stride(s.K) == cs && ((cs == 104 && nt == 7) || (cs == 8 && nt == 1));
+  const bool k3 = s.KH == 3 && s.KW == 3, k2 = s.KH == 2 && s.KW == 2;
+  if (!(k3 || k2) || feature_stride(s.K) != cs) return false;
+  return (cs == 104 && nt == 7) || (cs == 8 && nt == 1) || (k3 && cs == 64 && nt == 4);
 }
 
 static bool launch_mfma(const ConvShape& s, int precision, const void* x, const void* wpk, const float* b,
                         int act, void* y, hipStream_t st) {
   if (precision == CCS_BF16) {
     return try_bf16<3, 3, 13, 7>(s, x, wpk, b, act, y, st) || try_bf16<2, 2, 13, 7>(s, x, wpk, b, act, y, st) ||
-           try_bf16<3, 3, 1, 1>(s, x, wpk, b, act, y, st) || try_bf16<2, 2, 1, 1>(s, x, wpk, b, act, y, st);
+           try_bf16<3, 3, 8, 4>(s, x, wpk, b, act, y, st) || try_bf16<3, 3, 1, 1>(s, x, wpk, b, act, y, st) ||
+           try_bf16<2, 2, 1, 1>(s, x, wpk, b, act, y, st);
   }
   return try_f32<3, 3, 13, 7>(s, x, wpk, b, act, y, st) || try_f32<2, 2, 13, 7>(s, x, wpk, b, act, y, st) ||
-         try_f32<3, 3, 1, 1>(s, x, wpk, b, act, y, st) || try_f32<2, 2, 1, 1>(s, x, wpk, b, act, y, st);
+         try_f32<3, 3, 8, 4>(s, x, wpk, b, act, y, st) || try_f32<3, 3, 1, 1>(s, x, wpk, b, act, y, st) ||
+         try_f32<2, 2, 1, 1>(s, x, wpk, b, act, y, st);
 }
 
 static size_t packed_bytes(int K, int C, int KH, int KW, int precision) {

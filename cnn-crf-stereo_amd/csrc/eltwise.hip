@@ -105,9 +105,81 @@ __global__ void k_nchw_to_nhwc(const float* __restrict__ in, long HW, int C, int
   }
 }
 
+// Head of the pairwise network (stereo/stereonet.cpp:164, 1x1 convolution
+// without bias, then |.| at :402): out[k][p] = |sum_c w[k][c] x[p][c]| for
+// k < K, written as K planes of H x W (the 1 x 2 x H x W wx/wy buffer).  One
+// thread per pixel; x is channel-last with stride cs (a multiple of 8).
+template <typename T, int KMAX>
+__global__ __launch_bounds__(256) void k_pairwise_head(const T* __restrict__ x, long HW, int C, int cs,
+                                                       const float* __restrict__ w, int K, float* __restrict__ out) {
+  extern __shared__ float ws[];  // K x C
+  for (int i = threadIdx.x; i < K * C; i += blockDim.x) ws[i] = w[i];
+  __syncthreads();
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= HW) return;
+  float acc[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) acc[k] = 0.f;
+  const T* xp = x + p * cs;
+  for (int c0 = 0; c0 < C; c0 += 8) {  // 8 channels = one 16-byte (bf16) or two (fp32) loads
+    float v[8];
+    if constexpr (sizeof(T) == 2) {
+      const u32x4 t = *reinterpret_cast<const u32x4*>(xp + c0);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        v[2 * e] = __uint_as_float(t[e] << 16);
+        v[2 * e + 1] = __uint_as_float(t[e] & 0xffff0000u);
+      }
+    } else {
+      const float4 a = *reinterpret_cast<const float4*>(xp + c0), b = *reinterpret_cast<const float4*>(xp + c0 + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (k < K && c0 + e < C) acc[k] = fmaf(ws[k * C + c0 + e], v[e], acc[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k)
+    if (k < K) out[(long)k * HW + p] = fabsf(acc[k]);
+}
+
+// y *= s (iu::math::mulC, stereo/stereo.cpp:134-135).
+__global__ void k_scale(float* __restrict__ y, long n, float s) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] *= s;
+}
+
 }  // namespace ccs
 
 using namespace ccs;
+
+extern "C" int ccs_pairwise_head(ccs_handle_t h, const void* x, int precision, int H, int W, int C,
+                                 const float* w, int K, float* out) {
+  CCS_CHECK(h && x && w && out, CCS_ERR_ARG, "ccs_pairwise_head: NULL argument");
+  CCS_CHECK(H >= 1 && W >= 1 && C >= 1 && C <= 1024 && K >= 1 && K <= 4, CCS_ERR_ARG,
+            "ccs_pairwise_head: bad shape (H=%d W=%d C=%d K=%d; K <= 4)", H, W, C, K);
+  const long HW = (long)H * W;
+  const int cs = feature_stride(C);
+  const size_t lds = sizeof(float) * (size_t)K * C;
+  const unsigned grid = ceil_div(HW, 256);
+  if (precision == CCS_BF16)
+    k_pairwise_head<bf16, 4><<<grid, 256, lds, h->stream>>>((const bf16*)x, HW, C, cs, w, K, out);
+  else if (precision == CCS_FP32)
+    k_pairwise_head<float, 4><<<grid, 256, lds, h->stream>>>((const float*)x, HW, C, cs, w, K, out);
+  else
+    return fail(CCS_ERR_ARG, "ccs_pairwise_head: unknown precision %d", precision);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_scale_forward(ccs_handle_t h, float* y, long count, float s) {
+  CCS_CHECK(h && y && count >= 0, CCS_ERR_ARG, "ccs_scale_forward: bad argument");
+  if (count == 0) return CCS_OK;
+  k_scale<<<grid_for(count), kThreads, 0, h->stream>>>(y, count, s);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
 
 extern "C" int ccs_bias_forward(ccs_handle_t h, float* y, int layout, int C, int H, int W, int cs,
                                 const float* bias) {

@@ -22,6 +22,8 @@ namespace {
 
 [[noreturn]] void arg_fail(const std::string& msg) { throw Error(CCS_ERR_ARG, msg); }
 
+int prec_code(Precision p) { return p == Precision::BF16 ? CCS_BF16 : CCS_FP32; }
+
 int log2i(int num) {  // floor(log2(num)) for num >= 1, 0 for num <= 1 (stereo/stereo.cpp:56-62)
   int r = 0;
   while (num >>= 1) ++r;
@@ -41,15 +43,13 @@ StereoNet::StereoNet(int numLayers, int in, int ic, int ih, int iw)
 }
 
 StereoNet::~StereoNet() {
-  for (auto* v : {&m_leftOps, &m_rightOps, &m_lrOps})
+  for (auto* v : {&m_leftOps, &m_rightOps, &m_lrOps, &m_pairwiseOps})
     for (Operator* op : *v) delete op;
 }
 
 void StereoNet::initNet(float min_disp, float max_disp, float step, int rect_corr, Tensor* d_unaryOut,
                         Tensor* d_pairwiseOut) {
   if (m_initialized) arg_fail("StereoNet::initNet: already initialised");
-  if (d_pairwiseOut)
-    throw Error(CCS_ERR_UNSUPPORTED, "StereoNet::initNet: the pairwise CNN (COLORPAIR models) is not implemented");
   for (std::vector<Operator*>* ops : {&m_leftOps, &m_rightOps}) {
     // layer 1: 3x3, ic -> 100; layers 2..numLayers: 2x2, 100 -> 100 (stereo/stereonet.cpp:65-132)
     for (int l = 0; l < m_numLayers; ++l) {
@@ -70,7 +70,44 @@ void StereoNet::initNet(float min_disp, float max_disp, float step, int rect_cor
   auto* sm = new Softmax(corr->outTensorDesc(), Softmax::CHANNEL);
   sm->setNegate(true);
   m_lrOps.push_back(sm);
+  if (d_pairwiseOut) initPairwise(d_pairwiseOut);
   m_initialized = true;
+}
+
+// Pairwise network (stereo/stereonet.cpp:143-168): on the colour input, conv
+// 3x3 3->64 + bias + tanh, conv 3x3 64->64 + bias + tanh, conv 1x1 64->2
+// without bias, then |.| (:402) into d_pairwiseOut = (wx, wy).  The towers
+// shrink the padded input by numLayers + 1 pixels and this net by 4, so it
+// reads the input cropped by (numLayers - 3) / 2 pixels per side: 2 for seven
+// layers as in stereo/colorstereonet.cpp:69-73, none for three.
+void StereoNet::initPairwise(Tensor* d_pairwiseOut) {
+  if (m_ic != 3)
+    throw Error(CCS_ERR_UNSUPPORTED, "StereoNet::initNet: the pairwise network takes the 3-channel colour input");
+  m_pwCrop = (m_numLayers - 3) / 2;
+  const int ih = m_ih - 2 * m_pwCrop, iw = m_iw - 2 * m_pwCrop;
+  const TensorDesc& od = d_pairwiseOut->desc();
+  if (od.n != 1 || od.c != 2 || od.h != ih - 4 || od.w != iw - 4 || od.layout != Layout::NCHW ||
+      od.precision != Precision::FP32)
+    arg_fail("StereoNet::initNet: d_pairwiseOut must be a 1 x 2 x " + std::to_string(ih - 4) + " x " +
+             std::to_string(iw - 4) + " float32 planar tensor");
+  auto* c1 = new Convolution(Convolution::CONVOLUTION, m_in, 3, ih, iw, 64, 3, 3, 3);
+  m_pairwiseOps.push_back(c1);
+  c1->setPrecision(m_precision);
+  auto* b1 = new Bias(c1->outTensorDesc(), 1, 64, 1, 1);
+  m_pairwiseOps.push_back(b1);
+  m_pairwiseOps.push_back(new Activation(c1->outTensorDesc(), Activation::TANH));
+  c1->setFusedEpilogue(b1->getParams(), true);
+  auto* c2 = new Convolution(Convolution::CONVOLUTION, c1->outTensorDesc(), 64, 64, 3, 3);
+  m_pairwiseOps.push_back(c2);
+  c2->setPrecision(m_precision);
+  auto* b2 = new Bias(c2->outTensorDesc(), 1, 64, 1, 1);
+  m_pairwiseOps.push_back(b2);
+  m_pairwiseOps.push_back(new Activation(c2->outTensorDesc(), Activation::TANH));
+  c2->setFusedEpilogue(b2->getParams(), true);
+  auto* c3 = new Convolution(Convolution::CONVOLUTION, c2->outTensorDesc(), 2, 64, 1, 1);
+  m_pairwiseOps.push_back(c3);
+  c3->deleteOutputMemory();  // the head kernel writes d_pairwiseOut directly
+  m_pairwiseOut = d_pairwiseOut;
 }
 
 void StereoNet::setPrecision(Precision p) {
@@ -93,6 +130,16 @@ void StereoNet::setAllParams(const std::string& npzPath) {
     // The same array goes into both towers (shared weights, stereo/stereonet.cpp:196-206).
     m_leftOps[i]->setParams(it->second);
     m_rightOps[i]->setParams(it->second);
+    ++param_idx;
+  }
+  // Then the pairwise network's parameters, in the same walk (stereonet.cpp:209-224).
+  for (Operator* op : m_pairwiseOps) {
+    if (op->type() == Operator::ACTIVATION) continue;
+    const std::string key = "arr_" + std::to_string(param_idx);
+    auto it = all.find(key);
+    if (it == all.end()) throw Error(CCS_ERR_IO, npzPath + ": no array '" + key + "' (pairwise network)");
+    if (m_verbose) std::printf("PW: params_idx %d\n", param_idx);
+    op->setParams(it->second);
     ++param_idx;
   }
   // Trailing arrays (the learned L1, L2 of CRF models) are not network parameters.
@@ -123,6 +170,7 @@ Tensor* StereoNet::performPrediction(Tensor* d_inputLeft, Tensor* d_inputRight) 
   }
   std::vector<Tensor*> in{feats[0], feats[1]};
   Tensor* vol = m_lrOps[0]->forward(in, h);  // correlation + fused -softmax
+  if (!m_pairwiseOps.empty()) runPairwise(d_inputLeft);
   if (m_allowGc) {
     h.synchronize();
     for (auto* v : {&m_leftOps, &m_rightOps})
@@ -131,6 +179,37 @@ Tensor* StereoNet::performPrediction(Tensor* d_inputLeft, Tensor* d_inputRight) 
           if (op->type() == Operator::CONVOLUTION) static_cast<Convolution*>(op)->deleteOutputMemory();
   }
   return vol;
+}
+
+void StereoNet::runPairwise(Tensor* d_inputLeft) {
+  Handle& h = *m_handle;
+  Tensor* x = d_inputLeft;
+  std::unique_ptr<Tensor> cropped;
+  if (m_pwCrop > 0) {  // cuda::cropTensor (stereo/colorstereonet.cpp:71-72), per channel plane
+    const TensorDesc& d = d_inputLeft->desc();
+    const int ch = d.h - 2 * m_pwCrop, cw = d.w - 2 * m_pwCrop;
+    cropped.reset(new Tensor(TensorDesc(1, d.c, ch, cw)));
+    for (int c = 0; c < d.c; ++c) {
+      const float* src = d_inputLeft->data() + ((size_t)c * d.h + m_pwCrop) * d.w + m_pwCrop;
+      if (hipMemcpy2DAsync(cropped->data() + (size_t)c * ch * cw, sizeof(float) * cw, src, sizeof(float) * d.w,
+                           sizeof(float) * cw, ch, hipMemcpyDeviceToDevice, (hipStream_t)h.stream()) != hipSuccess)
+        throw Error(CCS_ERR_HIP, "StereoNet: cropping the pairwise input");
+    }
+    x = cropped.get();
+  }
+  x = static_cast<Convolution*>(m_pairwiseOps[0])->forward(x, h);  // + bias + tanh
+  Convolution* c2 = static_cast<Convolution*>(m_pairwiseOps[3]);
+  Tensor* y = c2->forward(x, h);                                   // + bias + tanh
+  const TensorDesc& yd = y->desc();
+  Convolution* c3 = static_cast<Convolution*>(m_pairwiseOps[6]);
+  check(ccs_pairwise_head(h.get(), y->raw(), prec_code(yd.precision), yd.h, yd.w, 64, c3->getParams()->data(), 2,
+                          m_pairwiseOut->data()),
+        "StereoNet: pairwise head");
+  if (cropped || m_allowGc) h.synchronize();  // the crop (and freed maps) must outlive the launches
+  if (m_allowGc) {
+    static_cast<Convolution*>(m_pairwiseOps[0])->deleteOutputMemory();
+    c2->deleteOutputMemory();
+  }
 }
 
 Tensor* StereoNet::predictPrepared(Tensor* d_left, Tensor* d_right) { return performPrediction(d_left, d_right); }
@@ -255,12 +334,14 @@ int Stereo::padval() const {
 void Stereo::ensureNet() {
   if (stereoNet_) return;
   const int p = padval();
-  const int ic = matching_function_ == COLORCNN ? 3 : 1;
+  const bool pairwise = matching_function_ == COLORCNN_PAIRWISE;
+  const int ic = matching_function_ == COLORCNN || pairwise ? 3 : 1;
   stereoNet_ = ic == 3 ? new ColorStereoNet(num_cnn_layers_, 1, 3, height_ + 2 * p, width_ + 2 * p)
                        : new StereoNet(num_cnn_layers_, 1, 1, height_ + 2 * p, width_ + 2 * p);
   stereoNet_->setPrecision(precision_);
   stereoNet_->setVerbose(verbose_);
-  stereoNet_->initNet(disp_min_, disp_max_, disp_step_, 0, costvolume_);
+  // COLORPAIR: the pairwise net writes wx, wy (stereo/stereo.cpp:240-247).
+  stereoNet_->initNet(disp_min_, disp_max_, disp_step_, 0, costvolume_, pairwise ? pairwise_ : nullptr);
   std::string params = cnn_parameter_file_base_name_;
   if (params.size() < 4 || params.compare(params.size() - 4, 4, ".npz") != 0) params += ".npz";
   stereoNet_->setAllParams(params);
@@ -270,13 +351,14 @@ void Stereo::ensureNet() {
 void Stereo::computeVolume(bool lr) {
   if (!img1_) arg_fail("Stereo: call initialize first");
   if (matching_function_ == LOAD) return;
-  if (matching_function_ != COLORCNN && matching_function_ != GRAYCNN)
+  const bool colour = matching_function_ == COLORCNN || matching_function_ == COLORCNN_PAIRWISE;
+  if (!colour && matching_function_ != GRAYCNN)
     throw Error(CCS_ERR_UNSUPPORTED, "Stereo::computeVolume: matching function not implemented");
-  if (matching_function_ == COLORCNN && !have_color_) arg_fail("Stereo: COLORCNN needs setColorImages");
+  if (colour && !have_color_) arg_fail("Stereo: COLORCNN / COLORPAIR need setColorImages");
   ensureNet();
   Handle& h = stereoNet_->handle();
   const int p = padval();
-  const int ic = matching_function_ == COLORCNN ? 3 : 1;
+  const int ic = colour ? 3 : 1;
   const int Hp = height_ + 2 * p, Wp = width_ + 2 * p;
   if (lr)
     stereoNet_->setDisparities(disp_min_, disp_max_, disp_step_, costvolume_);
@@ -315,11 +397,26 @@ void Stereo::computeWeights(float alpha, float beta, float lambda, bool lr) {
         "Stereo::computeWeights");
 }
 
+Tensor* Stereo::getWx(float alpha, float beta, bool left) {
+  if (alpha > 0) computeWeights(alpha, beta, 1.f, left);
+  handle_->synchronize();
+  return wx_;
+}
+
+Tensor* Stereo::getWy(float alpha, float beta, bool left) {
+  if (alpha > 0) computeWeights(alpha, beta, 1.f, left);
+  handle_->synchronize();
+  return wy_;
+}
+
 void Stereo::computeSlackPropagation(float C1, float C2, float delta, int iterations, float alpha, float beta,
                                      float lambda_reg, bool lr) {
   computeVolume(lr);
-  computeWeights(alpha, beta, lambda_reg, lr);
   Handle& h = *handle_;
+  if (matching_function_ == COLORCNN_PAIRWISE)  // weights from the CNN, times lambda (stereo/stereo.cpp:132-136)
+    check(ccs_scale_forward(h.get(), pairwise_->data(), 2L * width_ * height_, lambda_reg), "Stereo: scale weights");
+  else
+    computeWeights(alpha, beta, lambda_reg, lr);
   Tensor lb(TensorDesc(1, 1, 1, 2));
   check(ccs_slackprop(h.get(), costvolume_->data(), wx_->data(), wy_->data(), height_, width_, depth_, C1, C2,
                       (int)std::lround(delta), iterations, (int*)labels_->raw(), nullptr, nullptr, (double*)lb.raw()),

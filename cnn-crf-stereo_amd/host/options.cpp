@@ -23,7 +23,7 @@ const Spec kSpecs[] = {
     {"disp-min", "1", "Minimum disparity"},
     {"disp-step", "1", "Disparity step"},
     {"disp-max", "128", "Maximum disparity"},
-    {"matching", "CENSUS", "Matching method: COLORCNN, GRAYCNN, LOAD (CENSUS, COLORPAIR: not implemented)"},
+    {"matching", "CENSUS", "Matching method: COLORCNN, GRAYCNN, COLORPAIR, LOAD (CENSUS: not implemented)"},
     {"volume", "", "Volume file (.npy, H x W x D) when matching is LOAD"},
     {"inference", "CRF", "Inference method: CRF, ARGMAX, NONE"},
     {"refinement", "NONE", "Continuous refinement: NONE"},

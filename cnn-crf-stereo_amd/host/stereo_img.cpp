@@ -2,6 +2,7 @@
 // MI355X library.  Same options and config-file handling; outputs
 // <output-file>.png (16-bit disparity * 256), .pfm, _wx.pfm, _wy.pfm for CRF
 // inference, .png/.pfm for ARGMAX, and the .npy volume plus weights for NONE.
+// Matching: COLORCNN, GRAYCNN, COLORPAIR (pairwise CNN weights), LOAD.
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -42,22 +43,24 @@ static int run(int argc, char** argv) {
   stereo.setPrecision(prec == "bf16" ? Precision::BF16 : Precision::FP32);
 
   const std::string matching = o.str("matching");
-  if (matching == "COLORCNN" || matching == "GRAYCNN") {
+  float alpha = (float)o.num("alpha");
+  if (matching == "COLORCNN" || matching == "GRAYCNN" || matching == "COLORPAIR") {
     stereo.setNumLayers(o.integer("num-layers"));
     stereo.setParameterFile(o.str("parameter-file"));
-    stereo.setMatchingFunction(matching == "COLORCNN" ? COLORCNN : GRAYCNN);
+    stereo.setMatchingFunction(matching == "COLORCNN" ? COLORCNN : matching == "GRAYCNN" ? GRAYCNN : COLORCNN_PAIRWISE);
+    if (matching == "COLORPAIR") alpha = 0.f;  // the weights come from the pairwise CNN (stereo_img.cpp:151-156)
   } else if (matching == "LOAD") {
     stereo.setMatchingFunction(LOAD);
     if (!stereo.loadVolumeFromFile(o.str("volume")))
       throw Error(CCS_ERR_IO, "cannot load the volume " + o.str("volume"));
-  } else if (matching == "CENSUS" || matching == "COLORPAIR") {
+  } else if (matching == "CENSUS") {
     throw Error(CCS_ERR_UNSUPPORTED, "matching " + matching + " is not implemented by this build");
   } else {
     throw Error(CCS_ERR_ARG, "Unknown matching function " + matching);
   }
   stereo.setColorImages(I1.rgba.data(), I2.rgba.data());
 
-  const float alpha = (float)o.num("alpha"), beta = (float)o.num("beta"), lambda = (float)o.num("lambda");
+  const float beta = (float)o.num("beta"), lambda = (float)o.num("lambda");
   const std::string inference = o.str("inference");
   const auto t0 = std::chrono::steady_clock::now();
   if (inference == "CRF") {
@@ -65,7 +68,9 @@ static int run(int argc, char** argv) {
     stereo.computeSlackPropagation((float)o.num("L1"), (float)o.num("L2"), (float)o.num("delta"),
                                    o.integer("crf-iterations"), alpha, beta, lambda, true);
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    std::vector<float> d = to_host(stereo.getOutput()), wx = to_host(stereo.getWx()), wy = to_host(stereo.getWy());
+    // As the reference saves them: recomputed with lambda = 1 when alpha > 0 (stereo.cpp:425-437).
+    std::vector<float> d = to_host(stereo.getOutput()), wx = to_host(stereo.getWx(alpha, beta, true)),
+                       wy = to_host(stereo.getWy(alpha, beta, true));
     save_png_disparity(out + ".png", d.data(), I1.width, I1.height, true);
     save_pfm(out + ".pfm", d.data(), I1.width, I1.height);
     save_pfm(out + "_wx.pfm", wx.data(), I1.width, I1.height);
@@ -79,11 +84,10 @@ static int run(int argc, char** argv) {
     save_pfm(out + ".pfm", d.data(), I1.width, I1.height);
   } else if (inference == "NONE") {
     stereo.computeVolume(true);
-    stereo.computeWeights(alpha, beta, lambda, true);
     HostArray wx, wy, vol;
     wx.shape = wy.shape = {I1.height, I1.width};
-    wx.data = to_host(stereo.getWx());
-    wy.data = to_host(stereo.getWy());
+    wx.data = to_host(stereo.getWx(alpha, beta, true));
+    wy.data = to_host(stereo.getWy(alpha, beta, true));
     vol.shape = {I1.height, I1.width, stereo.getDepth()};
     vol.data = to_host(stereo.getVolume());
     npz::save_npy(out + "_wx.npy", wx);

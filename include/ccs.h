@@ -224,6 +224,17 @@ int ccs_bias_forward(ccs_handle_t h, float* y, int layout, int C, int H, int W, 
  * CUDNN_ACTIVATION_TANH (stereo/operators/activation.cpp:25,33-40). */
 int ccs_tanh_forward(ccs_handle_t h, float* y, long count);
 
+/* Head of the pairwise network: out[k][p] = |sum_c w[k][c] x[p][c]|, k < K <= 4,
+ * written as K planes of H x W floats (x channel-last, stride round_up(C, 8),
+ * fp32 or bf16; w the K x C x 1 x 1 kernel).  Replaces the 1x1 Convolution
+ * and the iu::math::abs of the pairwise net (stereo/stereonet.cpp:164,402). */
+int ccs_pairwise_head(ccs_handle_t h, const void* x, int precision, int H, int W, int C, const float* w, int K,
+                      float* out);
+
+/* y *= s over count floats (iu::math::mulC of the pairwise weights by lambda,
+ * stereo/stereo.cpp:134-135). */
+int ccs_scale_forward(ccs_handle_t h, float* y, long count, float s);
+
 /* Softmax over the D contiguous channels of each of HW pixels, in place
  * (max-subtracted); negate != 0 also applies cost = -p.  Replaces
  * Softmax::forward (stereo/operators/softmax.cpp:30-38, CHANNEL mode, NHWC)

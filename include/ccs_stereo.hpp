@@ -333,6 +333,9 @@ class StereoNet {
   void setAllParams(const std::string& npzPath);
   void setVerbose(bool val) { m_verbose = val; }
   void setAllowGc(bool value) { m_allowGc = value; }
+  // With d_pairwiseOut (a 1 x 2 x H x W float32 planar tensor, H x W the
+  // towers' output size) the network also builds the pairwise CNN of the
+  // COLORPAIR models (colour input only): performPrediction then writes wx, wy.
   virtual void initNet(float min_disp = 0, float max_disp = 0, float step = 0, int rect_corr = 0,
                        Tensor* d_unaryOut = nullptr, Tensor* d_pairwiseOut = nullptr);
   // MI355X additions: tower arithmetic (BF16 MFMA by default) and the handle the
@@ -340,6 +343,7 @@ class StereoNet {
   void setPrecision(Precision p);
   Handle& handle() { return *m_handle; }
   const std::vector<Operator*>& leftOps() const { return m_leftOps; }
+  const std::vector<Operator*>& pairwiseOps() const { return m_pairwiseOps; }
 
   // Gray (ic = 1) or planar prediction from padded, un-normalised device images
   // (C x ih x iw float32): normalises each image, runs both towers and the
@@ -351,9 +355,13 @@ class StereoNet {
 
  protected:
   Tensor* performPrediction(Tensor* d_inputLeft, Tensor* d_inputRight);
+  void initPairwise(Tensor* d_pairwiseOut);
+  void runPairwise(Tensor* d_inputLeft);
 
   std::unique_ptr<Handle> m_handle;
-  std::vector<Operator*> m_leftOps, m_rightOps, m_lrOps;
+  std::vector<Operator*> m_leftOps, m_rightOps, m_lrOps, m_pairwiseOps;
+  Tensor* m_pairwiseOut = nullptr;  // 1 x 2 x H x W: wx, wy (caller-owned)
+  int m_pwCrop = 0;                 // pixels cropped from each side of the pairwise net's input
   int m_numLayers, m_in, m_ic, m_ih, m_iw;
   bool m_verbose = false, m_initialized = false, m_allowGc = false;
   Precision m_precision = Precision::BF16;
@@ -397,6 +405,10 @@ class Stereo {
   void computeArgMax(bool lr);
   void computeVolume(bool lr);
   void computeWeights(float alpha, float beta, float lambda, bool lr);
+  // stereo/stereo.cpp:425-437: with alpha > 0 the edge weights are recomputed
+  // with lambda = 1 first; otherwise the current wx / wy are returned.
+  Tensor* getWx(float alpha, float beta, bool left);
+  Tensor* getWy(float alpha, float beta, bool left);
   void setMatchingFunction(MatchingFunction function) { matching_function_ = function; }
   void setNumLayers(int num_layers) { num_cnn_layers_ = num_layers; }
   void setParameterFile(const std::string& base_name) { cnn_parameter_file_base_name_ = base_name; }

@@ -43,7 +43,7 @@ def case(tmp_path_factory):
         feats.append(x)
     cost = O.neg_softmax(O.correlation(feats[0], feats[1], -np.arange(D)))
     gray = O.rgb2gray(imgs[0])
-    return dict(tmp=tmp, cost=cost, gray=gray)
+    return dict(tmp=tmp, cost=cost, gray=gray, imgs=imgs)
 
 
 def run_cli(case, *extra):
@@ -112,3 +112,76 @@ def test_cli_rejects_bad_options(case):
                         "--num-layers", "4", "--parameter-file", str(t / "params")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 2 and "3, 5 or 7" in r.stderr
+
+
+# ------------------------------------------------------------------ COLORPAIR
+
+def pairwise_oracle(img, arrays, pad, crop):
+    """The pairwise net in FP64: conv 3x3 3->64 + bias + tanh, conv 3x3 64->64 +
+    bias + tanh, conv 1x1 64->2 (no bias), |.|, on the normalised padded left
+    image cropped by `crop` pixels per side."""
+    x, _, _ = O.normalize(O.pad_symmetric(img, pad))
+    if crop:
+        x = x[:, crop:-crop, crop:-crop]
+    x = O.conv2d(x, arrays[0], arrays[1], True)
+    x = O.conv2d(x, arrays[2], arrays[3], True)
+    return np.abs(O.conv2d(x, arrays[4], None, False))
+
+
+@pytest.fixture(scope="module", params=[3, 7])
+def pair_case(request, case, tmp_path_factory):
+    layers_n = request.param
+    tmp = tmp_path_factory.mktemp("pair%d" % layers_n)
+    rng = np.random.default_rng(40 + layers_n)
+    layers, pad = synth.architecture(layers_n, 3, "reference")
+    arrays = []
+    for w, b in synth.glorot_weights(layers, seed=41 + layers_n):
+        arrays += [w, rng.uniform(-0.1, 0.1, b.shape).astype(np.float32)]
+    pw = [rng.uniform(-0.5, 0.5, (64, 3, 3, 3)), rng.uniform(-0.1, 0.1, 64),
+          rng.uniform(-0.15, 0.15, (64, 64, 3, 3)), rng.uniform(-0.1, 0.1, 64), rng.uniform(-0.3, 0.3, (2, 64, 1, 1))]
+    pw = [a.astype(np.float32) for a in pw]
+    np.savez(tmp / "params.npz", *arrays, *pw, np.float32([0.5]), np.float32([2.0]))
+    wxy = pairwise_oracle(case["imgs"][0], pw, pad, (layers_n - 3) // 2)
+    return dict(tmp=tmp, layers=layers_n, wxy=wxy, imgs=case["tmp"])
+
+
+def run_pair(pc, *extra):
+    t, it = pc["tmp"], pc["imgs"]
+    cmd = [CLI, str(it / "left.ppm"), str(it / "right.ppm"), "--matching", "COLORPAIR", "--num-layers",
+           str(pc["layers"]), "--parameter-file", str(t / "params"), "--disp-min", "0", "--disp-max", "15",
+           "--disp-step", "1", "--precision", "fp32", *map(str, extra)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_cli_colorpair_weights(pair_case):
+    """wx, wy of the pairwise CNN (3- and 7-layer models: no crop / 2-pixel crop)."""
+    out = pair_case["tmp"] / "pw"
+    run_pair(pair_case, "--inference", "NONE", "-o", out)
+    wx, wy = np.load(str(out) + "_wx.npy"), np.load(str(out) + "_wy.npy")
+    assert wx.shape == (H, W)
+    np.testing.assert_allclose(wx, pair_case["wxy"][0], atol=1e-4, rtol=0)
+    np.testing.assert_allclose(wy, pair_case["wxy"][1], atol=1e-4, rtol=0)
+
+
+def test_cli_colorpair_crf(pair_case):
+    """CRF on the CNN weights times lambda (the edge-image term is off)."""
+    out = pair_case["tmp"] / "pwcrf"
+    log = run_pair(pair_case, "--inference", "CRF", "--L1", "0.5", "--L2", "2", "--delta", "2",
+                   "--crf-iterations", "3", "--lambda", "1.5", "-o", out)
+    b = open(str(out) + ".pfm", "rb").read()
+    hdr = b"Pf\n%d %d\n-1.0\n" % (W, H)
+    disp = np.frombuffer(b[len(hdr):], dtype="<f4").reshape(H, W)[::-1]
+    vol_path = pair_case["tmp"] / "pwvol"
+    run_pair(pair_case, "--inference", "NONE", "-o", vol_path)
+    vol = np.load(str(vol_path) + ".npy").astype(np.float64)
+    wx = (1.5 * np.load(str(vol_path) + "_wx.npy")).astype(np.float32)
+    wy = (1.5 * np.load(str(vol_path) + "_wy.npy")).astype(np.float32)
+    r = O.slackprop(vol, wx, wy, 0.5, 2.0, 2, 3)
+    mism = disp.astype(np.int64) != r["labels"]
+    assert np.all(r["gap"][mism] < 1e-3), "COLORPAIR disparities differ from the oracle at decided pixels"
+    lb = float(log.split("lower bound")[1].split(",")[0])
+    assert abs(lb - r["lb"]) <= 1e-5 * max(1.0, abs(r["lb"]))
+    saved = np.frombuffer(open(str(out) + "_wx.pfm", "rb").read()[len(hdr):], dtype="<f4").reshape(H, W)[::-1]
+    np.testing.assert_allclose(saved, wx, rtol=1e-6, atol=0)  # the weights the CRF used (lambda applied)

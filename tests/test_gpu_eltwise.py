@@ -90,3 +90,31 @@ def test_c4_to_planar(handle):
     call("ccs_c4_to_planar", handle.h, ptr(img), H, W, pitch, 3, ptr(out))
     want = img[:, :4 * W].reshape(H, W, 4)[:, :, :3].permute(2, 0, 1)
     torch.testing.assert_close(out, want, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("C,K", [(64, 2), (5, 3), (17, 1)])
+def test_pairwise_head(handle, precision, C, K):
+    """|1x1 convolution| into K planes, against numpy float64 on the same
+    (bf16-rounded) inputs; 1e-5 relative to the sum of |terms|."""
+    rng = np.random.default_rng(C + K)
+    H, W = 7, 45
+    cs = (C + 7) // 8 * 8
+    dt = torch.bfloat16 if precision == "bf16" else torch.float32
+    x = torch.zeros(H, W, cs, dtype=dt, device="cuda")
+    x[:, :, :C] = torch.as_tensor(rng.uniform(-1, 1, (H, W, C)), dtype=torch.float32, device="cuda").to(dt)
+    w = torch.as_tensor(rng.uniform(-0.5, 0.5, (K, C)), dtype=torch.float32, device="cuda")
+    out = torch.empty(K, H, W, device="cuda")
+    call("ccs_pairwise_head", handle.h, ptr(x), BF16 if precision == "bf16" else FP32, H, W, C, ptr(w), K, ptr(out))
+    xf = x[:, :, :C].double().cpu().numpy()
+    wf = w.double().cpu().numpy()
+    want = np.abs(np.einsum("hwc,kc->khw", xf, wf))
+    scale = np.einsum("hwc,kc->khw", np.abs(xf), np.abs(wf))
+    assert np.all(np.abs(out.cpu().numpy() - want) <= 1e-5 * scale + 1e-7)
+
+
+def test_scale_forward(handle):
+    y = torch.linspace(-3, 3, 1001, device="cuda")
+    want = y * 0.37
+    call("ccs_scale_forward", handle.h, ptr(y), y.numel(), 0.37)
+    torch.testing.assert_close(y, want, rtol=0, atol=0)

@@ -70,7 +70,7 @@ def test_edge_weights(handle):
 # ---------------------------------------------------------------- convolution
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
-@pytest.mark.parametrize("C,K,k", [(100, 100, 3), (100, 100, 2), (8, 8, 3), (8, 8, 2), (5, 12, 3)])
+@pytest.mark.parametrize("C,K,k", [(100, 100, 3), (100, 100, 2), (64, 64, 3), (8, 8, 3), (8, 8, 2), (5, 12, 3)])
 def test_conv_layer(handle, precision, C, K, k):
     rng = np.random.default_rng(10 + C + K + k)
     H, W = 9, 70 + k
