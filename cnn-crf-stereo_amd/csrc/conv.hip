@@ -414,7 +414,7 @@ __global__ void k_conv_generic(const TI* __restrict__ x, int x_nchw, int Hin, in
 }
 
 // ---------------------------------------------------------------------------
-// Image layer: C <= 4 input channels (C x H x W float32, the normalised padded
+// Image layer: C <= 5 input channels (C x H x W float32, the normalised padded
 // image), 3x3, K <= 128 outputs.  One thread per output pixel computes every
 // output channel; the flipped weights sit in LDS and the block's 256 x Cs output
 // tile is staged in LDS for contiguous stores.
@@ -615,12 +615,13 @@ static int conv_generic(ccs_handle_t h, const void* x, int x_layout, int H, int 
                         const float* b, int K, int KH, int KW, int act, int precision, void* y) {
   const int Ho = H - KH + 1, Wo = W - KW + 1;
   const int Cs_out = feature_stride(K);
-  if (x_layout == CCS_LAYOUT_NCHW && KH == 3 && KW == 3 && K <= 128 && C <= 4) {
+  if (x_layout == CCS_LAYOUT_NCHW && KH == 3 && KW == 3 && K <= 128 && C <= 5) {
     switch (C) {
       case 1: launch_image<1>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
       case 2: launch_image<2>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
       case 3: launch_image<3>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
-      default: launch_image<4>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
+      case 4: launch_image<4>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
+      default: launch_image<5>((const float*)x, H, W, w, b, K, act, precision, y, h->stream); break;
     }
     CCS_LAUNCH_CHECK();
     return CCS_OK;

@@ -145,6 +145,18 @@ __global__ __launch_bounds__(256) void k_pairwise_head(const T* __restrict__ x, 
     if (k < K) out[(long)k * HW + p] = fabsf(acc[k]);
 }
 
+// Position planes of the x/y-input model (stereo/colorstereonet.cpp:101-112):
+// out[0][y][x] = (x - W/2) / W, out[1][y][x] = (y - H/2) / H, evaluated in
+// double as the reference's float/double expression is, then rounded to float.
+__global__ void k_xy_planes(int H, int W, float* __restrict__ out) {
+  const long n = (long)H * W;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int y = (int)(i / W), x = (int)(i - (long)y * W);
+    out[i] = (float)(((double)(float)x - (double)(float)W / 2.0) / (double)(float)W);
+    out[n + i] = (float)(((double)(float)y - (double)(float)H / 2.0) / (double)(float)H);
+  }
+}
+
 // y *= s (iu::math::mulC, stereo/stereo.cpp:134-135).
 __global__ void k_scale(float* __restrict__ y, long n, float s) {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] *= s;
@@ -169,6 +181,13 @@ extern "C" int ccs_pairwise_head(ccs_handle_t h, const void* x, int precision, i
     k_pairwise_head<float, 4><<<grid, 256, lds, h->stream>>>((const float*)x, HW, C, cs, w, K, out);
   else
     return fail(CCS_ERR_ARG, "ccs_pairwise_head: unknown precision %d", precision);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_xy_planes(ccs_handle_t h, int H, int W, float* out) {
+  CCS_CHECK(h && out && H >= 1 && W >= 1, CCS_ERR_ARG, "ccs_xy_planes: bad argument");
+  k_xy_planes<<<grid_for((long)H * W), kThreads, 0, h->stream>>>(H, W, out);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }

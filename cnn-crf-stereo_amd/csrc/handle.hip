@@ -29,7 +29,7 @@ int Scratch::reserve(size_t need) {
     // The buffer may still be read by queued work of this handle.
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail(CCS_ERR_HIP, "hipDeviceSynchronize: %s", hipGetErrorString(e));
-    hipFree(ptr);
+    (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
   }
@@ -44,7 +44,7 @@ int Scratch::reserve(size_t need) {
 }
 
 void Scratch::release() {
-  if (ptr) hipFree(ptr);
+  if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
 }
@@ -70,7 +70,7 @@ extern "C" int ccs_create(ccs_handle_t* handle, void* hip_stream) {
 
 extern "C" int ccs_destroy(ccs_handle_t h) {
   if (!h) return CCS_OK;
-  hipStreamSynchronize(h->stream);
+  (void)hipStreamSynchronize(h->stream);
   h->conv_w.release();
   h->crf_a.release();
   h->crf_msg.release();

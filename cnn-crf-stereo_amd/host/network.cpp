@@ -39,7 +39,7 @@ StereoNet::StereoNet(int numLayers, int in, int ic, int ih, int iw)
   if (numLayers != 3 && numLayers != 5 && numLayers != 7)
     arg_fail("StereoNet: numLayers must be 3, 5 or 7 (stereo/stereo.cpp:198-202)");
   if (in != 1) throw Error(CCS_ERR_UNSUPPORTED, "StereoNet: batch size 1 only");
-  if (ic < 1 || ic > 4) throw Error(CCS_ERR_UNSUPPORTED, "StereoNet: 1 to 4 input channels");
+  if (ic < 1 || ic > 5) throw Error(CCS_ERR_UNSUPPORTED, "StereoNet: 1 to 5 input channels");
 }
 
 StereoNet::~StereoNet() {
@@ -81,8 +81,8 @@ void StereoNet::initNet(float min_disp, float max_disp, float step, int rect_cor
 // reads the input cropped by (numLayers - 3) / 2 pixels per side: 2 for seven
 // layers as in stereo/colorstereonet.cpp:69-73, none for three.
 void StereoNet::initPairwise(Tensor* d_pairwiseOut) {
-  if (m_ic != 3)
-    throw Error(CCS_ERR_UNSUPPORTED, "StereoNet::initNet: the pairwise network takes the 3-channel colour input");
+  if (m_ic != 3 && m_ic != 5)
+    throw Error(CCS_ERR_UNSUPPORTED, "StereoNet::initNet: the pairwise network takes the colour input");
   m_pwCrop = (m_numLayers - 3) / 2;
   const int ih = m_ih - 2 * m_pwCrop, iw = m_iw - 2 * m_pwCrop;
   const TensorDesc& od = d_pairwiseOut->desc();
@@ -183,13 +183,16 @@ Tensor* StereoNet::performPrediction(Tensor* d_inputLeft, Tensor* d_inputRight) 
 
 void StereoNet::runPairwise(Tensor* d_inputLeft) {
   Handle& h = *m_handle;
-  Tensor* x = d_inputLeft;
+  // The pairwise net reads the RGB planes only (with the x/y model the input
+  // has two more planes after them, stereo/colorstereonet.cpp:127-130).
+  const TensorDesc& d = d_inputLeft->desc();
+  Tensor rgb(d_inputLeft->data(), TensorDesc(1, 3, d.h, d.w));
+  Tensor* x = &rgb;
   std::unique_ptr<Tensor> cropped;
   if (m_pwCrop > 0) {  // cuda::cropTensor (stereo/colorstereonet.cpp:71-72), per channel plane
-    const TensorDesc& d = d_inputLeft->desc();
     const int ch = d.h - 2 * m_pwCrop, cw = d.w - 2 * m_pwCrop;
-    cropped.reset(new Tensor(TensorDesc(1, d.c, ch, cw)));
-    for (int c = 0; c < d.c; ++c) {
+    cropped.reset(new Tensor(TensorDesc(1, 3, ch, cw)));
+    for (int c = 0; c < 3; ++c) {
       const float* src = d_inputLeft->data() + ((size_t)c * d.h + m_pwCrop) * d.w + m_pwCrop;
       if (hipMemcpy2DAsync(cropped->data() + (size_t)c * ch * cw, sizeof(float) * cw, src, sizeof(float) * d.w,
                            sizeof(float) * cw, ch, hipMemcpyDeviceToDevice, (hipStream_t)h.stream()) != hipSuccess)
@@ -228,20 +231,24 @@ Tensor* StereoNet::predictPlanar(const float* d_left, const float* d_right) {
 // ----------------------------------------------------------- ColorStereoNet
 
 ColorStereoNet::ColorStereoNet(int numLayers, int in, int ic, int ih, int iw) : StereoNet(numLayers, in, ic, ih, iw) {
-  if (ic == 5) throw Error(CCS_ERR_UNSUPPORTED, "ColorStereoNet: the x/y-input (5-channel) model is not implemented");
-  if (ic != 3) arg_fail("ColorStereoNet: ic must be 3");
+  if (ic != 3 && ic != 5) arg_fail("ColorStereoNet: ic must be 3 (RGB) or 5 (RGB + x/y position)");
 }
 
 Tensor* ColorStereoNet::predict(const float* d_left_c4, const float* d_right_c4, long pitch) {
   Handle& h = *m_handle;
   if (pitch == 0) pitch = 4L * m_iw;
-  TensorDesc d(1, 3, m_ih, m_iw);
+  TensorDesc d(1, m_ic, m_ih, m_iw);
   Tensor l(d), r(d);
   // C4 -> planar RGB, then per-channel zero mean / unit std (stereo/colorstereonet.cpp:53-68).
   check(ccs_c4_to_planar(h.get(), d_left_c4, m_ih, m_iw, pitch, 3, l.data()), "ColorStereoNet: unpack");
   check(ccs_c4_to_planar(h.get(), d_right_c4, m_ih, m_iw, pitch, 3, r.data()), "ColorStereoNet: unpack");
   check(ccs_pad_normalize(h.get(), l.data(), 3, m_ih, m_iw, 0, l.data(), nullptr), "ColorStereoNet: normalise");
   check(ccs_pad_normalize(h.get(), r.data(), 3, m_ih, m_iw, 0, r.data(), nullptr), "ColorStereoNet: normalise");
+  if (m_ic == 5) {  // x/y position planes after the normalised RGB (stereo/colorstereonet.cpp:100-124)
+    const size_t plane = (size_t)m_ih * m_iw;
+    check(ccs_xy_planes(h.get(), m_ih, m_iw, l.data() + 3 * plane), "ColorStereoNet: x/y planes");
+    check(ccs_xy_planes(h.get(), m_ih, m_iw, r.data() + 3 * plane), "ColorStereoNet: x/y planes");
+  }
   Tensor* out = performPrediction(&l, &r);
   h.synchronize();
   return out;
@@ -319,7 +326,8 @@ void Stereo::setColorImages(const float* img1_c4, const float* img2_c4) {
 }
 
 void Stereo::setWithXy(bool value) {
-  if (value) throw Error(CCS_ERR_UNSUPPORTED, "Stereo: the x/y-input model is not implemented");
+  if (stereoNet_ && value != with_xy_) arg_fail("Stereo::setWithXy: the network is already built");
+  with_xy_ = value;
 }
 
 int Stereo::padval() const {
@@ -335,9 +343,11 @@ void Stereo::ensureNet() {
   if (stereoNet_) return;
   const int p = padval();
   const bool pairwise = matching_function_ == COLORCNN_PAIRWISE;
-  const int ic = matching_function_ == COLORCNN || pairwise ? 3 : 1;
-  stereoNet_ = ic == 3 ? new ColorStereoNet(num_cnn_layers_, 1, 3, height_ + 2 * p, width_ + 2 * p)
-                       : new StereoNet(num_cnn_layers_, 1, 1, height_ + 2 * p, width_ + 2 * p);
+  const bool colour = matching_function_ == COLORCNN || pairwise;
+  // Colour models take RGB, plus the x/y position planes with with_xy (stereo/stereo.cpp:237-239).
+  const int ic = colour ? (with_xy_ ? 5 : 3) : 1;
+  stereoNet_ = colour ? new ColorStereoNet(num_cnn_layers_, 1, ic, height_ + 2 * p, width_ + 2 * p)
+                      : new StereoNet(num_cnn_layers_, 1, 1, height_ + 2 * p, width_ + 2 * p);
   stereoNet_->setPrecision(precision_);
   stereoNet_->setVerbose(verbose_);
   // COLORPAIR: the pairwise net writes wx, wy (stereo/stereo.cpp:240-247).
@@ -358,7 +368,8 @@ void Stereo::computeVolume(bool lr) {
   ensureNet();
   Handle& h = stereoNet_->handle();
   const int p = padval();
-  const int ic = colour ? 3 : 1;
+  const int ic = colour ? 3 : 1;  // image planes; the x/y model adds two position planes
+  const int nc = colour && with_xy_ ? 5 : ic;
   const int Hp = height_ + 2 * p, Wp = width_ + 2 * p;
   if (lr)
     stereoNet_->setDisparities(disp_min_, disp_max_, disp_step_, costvolume_);
@@ -380,12 +391,17 @@ void Stereo::computeVolume(bool lr) {
     }
   }
   if (!pad1_) {
-    pad1_ = new Tensor(TensorDesc(1, ic, Hp, Wp));
-    pad2_ = new Tensor(TensorDesc(1, ic, Hp, Wp));
+    pad1_ = new Tensor(TensorDesc(1, nc, Hp, Wp));
+    pad2_ = new Tensor(TensorDesc(1, nc, Hp, Wp));
   }
   // Mirror pad + per-channel normalisation in one pass each.
   check(ccs_pad_normalize(h.get(), src[0]->data(), ic, height_, width_, p, pad1_->data(), nullptr), "Stereo: pad");
   check(ccs_pad_normalize(h.get(), src[1]->data(), ic, height_, width_, p, pad2_->data(), nullptr), "Stereo: pad");
+  if (nc == 5) {  // position planes of the padded image (stereo/colorstereonet.cpp:100-124)
+    const size_t plane = (size_t)Hp * Wp;
+    check(ccs_xy_planes(h.get(), Hp, Wp, pad1_->data() + 3 * plane), "Stereo: x/y planes");
+    check(ccs_xy_planes(h.get(), Hp, Wp, pad2_->data() + 3 * plane), "Stereo: x/y planes");
+  }
   stereoNet_->predictPrepared(pad1_, pad2_);
   h.synchronize();
 }

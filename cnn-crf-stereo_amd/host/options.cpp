@@ -34,7 +34,7 @@ const Spec kSpecs[] = {
     {"config-file", nullptr, "Provide all options in a config file"},
     {"num-layers", "7", "Number of layers for CNN matching (3, 5, 7)"},
     {"parameter-file", "params", "Parameters for CNN matching (.npz, extension optional)"},
-    {"with-xy", "false", "Use x- and y-coordinates (not implemented)"},
+    {"with-xy", "false", "Use x- and y-coordinates (5-channel colour models)"},
     {"filter-size", "7", "Filter size for Census matching"},
     {"lambda", "0.4", "Weighting of regularization term"},
     {"L1", "1", "Parameter for CRF inference"},

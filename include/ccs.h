@@ -231,6 +231,11 @@ int ccs_tanh_forward(ccs_handle_t h, float* y, long count);
 int ccs_pairwise_head(ccs_handle_t h, const void* x, int precision, int H, int W, int C, const float* w, int K,
                       float* out);
 
+/* The two position planes of the x/y-input ColorStereoNet (out: 2 x H x W
+ * floats, x plane then y plane): (x - W/2) / W and (y - H/2) / H on the padded
+ * image (stereo/colorstereonet.cpp:101-112). */
+int ccs_xy_planes(ccs_handle_t h, int H, int W, float* out);
+
 /* y *= s over count floats (iu::math::mulC of the pairwise weights by lambda,
  * stereo/stereo.cpp:134-135). */
 int ccs_scale_forward(ccs_handle_t h, float* y, long count, float s);

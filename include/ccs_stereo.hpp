@@ -437,7 +437,7 @@ class Stereo {
   void ensureNet();
 
   std::unique_ptr<Handle> handle_;
-  bool verbose_ = false, allowCnnGc_ = false, have_color_ = false;
+  bool verbose_ = false, allowCnnGc_ = false, have_color_ = false, with_xy_ = false;
   int width_ = 0, height_ = 0, depth_ = 0;
   float disp_min_ = 0, disp_step_ = 1, disp_max_ = 0;
   int num_cnn_layers_ = 3;

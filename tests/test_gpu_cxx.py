@@ -185,3 +185,50 @@ def test_cli_colorpair_crf(pair_case):
     assert abs(lb - r["lb"]) <= 1e-5 * max(1.0, abs(r["lb"]))
     saved = np.frombuffer(open(str(out) + "_wx.pfm", "rb").read()[len(hdr):], dtype="<f4").reshape(H, W)[::-1]
     np.testing.assert_allclose(saved, wx, rtol=1e-6, atol=0)  # the weights the CRF used (lambda applied)
+
+
+# ------------------------------------------------------------- x/y-input model
+
+def xy_planes(Hp, Wp):
+    """(x - W/2) / W and (y - H/2) / H of the padded image, float64 then float32."""
+    x = ((np.arange(Wp, dtype=np.float64) - Wp / 2.0) / Wp).astype(np.float32)
+    y = ((np.arange(Hp, dtype=np.float64) - Hp / 2.0) / Hp).astype(np.float32)
+    return np.stack([np.broadcast_to(x[None, :], (Hp, Wp)), np.broadcast_to(y[:, None], (Hp, Wp))])
+
+
+@pytest.mark.parametrize("matching,layers_n", [("COLORCNN", 3), ("COLORPAIR", 7)])
+def test_cli_with_xy(case, tmp_path, matching, layers_n):
+    """5-channel towers (normalised RGB + position planes); the pairwise net of
+    the +xy model still reads the RGB planes only."""
+    rng = np.random.default_rng(60 + layers_n)
+    layers, pad = synth.architecture(layers_n, 5, "reference")
+    arrays = []
+    for w, b in synth.glorot_weights(layers, seed=61 + layers_n):
+        arrays += [w, rng.uniform(-0.1, 0.1, b.shape).astype(np.float32)]
+    pw = []
+    if matching == "COLORPAIR":
+        pw = [rng.uniform(-0.5, 0.5, (64, 3, 3, 3)), rng.uniform(-0.1, 0.1, 64),
+              rng.uniform(-0.15, 0.15, (64, 64, 3, 3)), rng.uniform(-0.1, 0.1, 64),
+              rng.uniform(-0.3, 0.3, (2, 64, 1, 1))]
+        pw = [a.astype(np.float32) for a in pw]
+    np.savez(tmp_path / "params.npz", *arrays, *pw)
+    t = case["tmp"]
+    out = tmp_path / "xy"
+    cmd = [CLI, str(t / "left.ppm"), str(t / "right.ppm"), "--matching", matching, "--num-layers", str(layers_n),
+           "--with-xy", "true", "--parameter-file", str(tmp_path / "params"), "--disp-min", "0", "--disp-max", "15",
+           "--disp-step", "1", "--precision", "fp32", "--inference", "NONE", "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    feats = []
+    for img in case["imgs"]:
+        x, _, _ = O.normalize(O.pad_symmetric(img, pad))
+        x = np.concatenate([x, xy_planes(H + 2 * pad, W + 2 * pad).astype(np.float64)])
+        for i in range(layers_n):
+            x = O.conv2d(x, arrays[2 * i], arrays[2 * i + 1], True)
+        feats.append(x)
+    cost = O.neg_softmax(O.correlation(feats[0], feats[1], -np.arange(D)))
+    np.testing.assert_allclose(np.load(str(out) + ".npy"), cost, atol=1e-4, rtol=0)
+    if matching == "COLORPAIR":
+        wxy = pairwise_oracle(case["imgs"][0], pw, pad, (layers_n - 3) // 2)
+        np.testing.assert_allclose(np.load(str(out) + "_wx.npy"), wxy[0], atol=1e-4, rtol=0)
+        np.testing.assert_allclose(np.load(str(out) + "_wy.npy"), wxy[1], atol=1e-4, rtol=0)
