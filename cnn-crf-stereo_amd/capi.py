@@ -52,6 +52,8 @@ SIGNATURES = {
     "ccs_pairwise_head": (_I, [_P, _P, _I, _I, _I, _I, _P, _I, _P]),
     "ccs_scale_forward": (_I, [_P, _P, ctypes.c_long, ctypes.c_float]),
     "ccs_xy_planes": (_I, [_P, _I, _I, _P]),
+    "ccs_quad_fit_direct": (_I, [_P, _P, _P, _I, _I, _I, _F, _F]),
+    "ccs_left_right_check": (_I, [_P, _P, _P, _I, _I, _F, _P]),
     "ccs_softmax_forward": (_I, [_P, _P, ctypes.c_long, _I, _I]),
     "ccs_neg_softmax_backward": (_I, [_P, _P, _P, ctypes.c_long, _I, _P]),
     "ccs_c4_to_planar": (_I, [_P, _P, _I, _I, ctypes.c_long, _I, _P]),

@@ -168,6 +168,49 @@ __global__ void k_remap(const int* __restrict__ lab, long n, float dmin, float d
   if (i < n) out[i] = (float)lab[i] * dstep + dmin;
 }
 
+// QuadDirect sub-pixel refinement (stereo/stereo.cu:924-942, 1172-1195): the
+// parabola through the three volume values around the pixel's label
+// z = (int)((d - disp_min) / disp_step), with h = disp_step:
+//   g = (c[z-1] - c[z+1]) / (2h),  k = max(1e-3, (c[z-1] + c[z+1] - 2 c[z]) / h^2),
+//   d += g / k.
+// Pixels with z <= 0 or z >= D - 1 keep d (the reference skips z = 0; at
+// z = D - 1 it would read one label past the pixel).
+__global__ void k_quad_fit(float* __restrict__ disp, const float* __restrict__ vol, long HW, int D, float dmin,
+                           float dstep) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= HW) return;
+  const float d = disp[i];
+  const int z = (int)((d - dmin) / dstep);
+  if (z <= 0 || z >= D - 1) return;
+  const float* c = vol + i * D + z;
+  const float g = (c[-1] - c[1]) / (dstep * 2.f);
+  const float k = fmaxf(1e-3f, (c[-1] + c[1] - 2.f * c[0]) / (dstep * dstep));
+  disp[i] = d + g / k;
+}
+
+// Left-right consistency (stereo/stereo.cu:803-817, 1198-1225): out = I1 where
+// |I1 + R(x + 0.5 - I1)| <= th, else -1, I1 the left disparity and R the right
+// one (negative disparities), sampled as the reference's linear, clamped
+// texture: at texel coordinate u, R = (1 - a) R[i] + a R[i + 1] with
+// i = floor(u - 0.5), a = u - 0.5 - i and indices clamped to the row.  (The
+// texture unit rounds a to 8 fractional bits; here it is exact.)
+__global__ void k_lr_check(const float* __restrict__ left, const float* __restrict__ right, int H, int W, float th,
+                           float* __restrict__ out) {
+  const long n = (long)H * W;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int y = (int)(i / W), x = (int)(i - (long)y * W);
+  const float I1 = left[i];
+  const float u = ((float)x + 0.5f - I1) - 0.5f;
+  const float fi = floorf(u);
+  const float a = u - fi;
+  const int i0 = (int)fminf(fmaxf(fi, 0.f), (float)(W - 1));
+  const int i1 = (int)fminf(fmaxf(fi + 1.f, 0.f), (float)(W - 1));
+  const float* r = right + (long)y * W;
+  const float I2 = -((1.f - a) * r[i0] + a * r[i1]);
+  out[i] = fabsf(I1 - I2) <= th ? I1 : -1.f;
+}
+
 }  // namespace ccs
 
 using namespace ccs;
@@ -230,6 +273,25 @@ extern "C" int ccs_argmin(ccs_handle_t h, const float* cost, int H, int W, int D
   } else {
     k_argmin<8><<<ceil_div(HW * 8, 256), 256, 0, h->stream>>>(cost, HW, D, disp_min, disp_step, out);
   }
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_quad_fit_direct(ccs_handle_t h, float* disparity, const float* volume, int H, int W, int D,
+                                   float disp_min, float disp_step) {
+  CCS_CHECK(h && disparity && volume && H > 0 && W > 0 && D > 0, CCS_ERR_ARG, "ccs_quad_fit_direct: bad argument");
+  CCS_CHECK(disp_step != 0.f, CCS_ERR_ARG, "ccs_quad_fit_direct: zero disparity step");
+  const long HW = (long)H * W;
+  k_quad_fit<<<ceil_div(HW, 256), 256, 0, h->stream>>>(disparity, volume, HW, D, disp_min, disp_step);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_left_right_check(ccs_handle_t h, const float* left, const float* right, int H, int W, float th,
+                                    float* out) {
+  CCS_CHECK(h && left && right && out && H > 0 && W > 0, CCS_ERR_ARG, "ccs_left_right_check: bad argument");
+  const long n = (long)H * W;
+  k_lr_check<<<ceil_div(n, 256), 256, 0, h->stream>>>(left, right, H, W, th, out);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }

@@ -276,7 +276,8 @@ Stereo::Stereo() = default;
 
 Stereo::~Stereo() {
   delete stereoNet_;
-  for (Tensor* t : {img1_, img2_, img1_color_, img2_color_, costvolume_, pairwise_, wx_, wy_, out_, labels_, pad1_, pad2_})
+  for (Tensor* t : {img1_, img2_, img1_color_, img2_color_, costvolume_, pairwise_, wx_, wy_, out_, labels_, pad1_, pad2_,
+                    c1_})
     delete t;
 }
 
@@ -434,9 +435,13 @@ void Stereo::computeSlackPropagation(float C1, float C2, float delta, int iterat
   else
     computeWeights(alpha, beta, lambda_reg, lr);
   Tensor lb(TensorDesc(1, 1, 1, 2));
+  if (!c1_) c1_ = new Tensor(TensorDesc(1, depth_, height_, width_, Layout::NHWC));
   check(ccs_slackprop(h.get(), costvolume_->data(), wx_->data(), wy_->data(), height_, width_, depth_, C1, C2,
-                      (int)std::lround(delta), iterations, (int*)labels_->raw(), nullptr, nullptr, (double*)lb.raw()),
+                      (int)std::lround(delta), iterations, (int*)labels_->raw(), c1_->data(), nullptr,
+                      (double*)lb.raw()),
         "Stereo::computeSlackPropagation");
+  c1_valid_ = true;
+  c1_lr_ = lr;
   // remapSolution (stereo/stereo.cpp:155-158): lr -> min + l step; rl -> -max + l step.
   check(ccs_remap(h.get(), (const int*)labels_->raw(), height_, width_, lr ? disp_min_ : -disp_max_, disp_step_,
                   out_->data()),
@@ -445,6 +450,25 @@ void Stereo::computeSlackPropagation(float C1, float C2, float delta, int iterat
   if (hipMemcpy(&lower_bound_, lb.raw(), sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     throw Error(CCS_ERR_HIP, "Stereo: reading the bound");
   if (verbose_) std::printf("CRF lower bound: %.6f\n", lower_bound_);
+}
+
+void Stereo::fuseQuadFitDirect() {
+  if (!c1_valid_) arg_fail("Stereo::fuseQuadFitDirect: call computeSlackPropagation first");
+  // The labels of an rl run start at -disp_max (the reference offsets by disp_min in both cases).
+  check(ccs_quad_fit_direct(handle_->get(), out_->data(), c1_->data(), height_, width_, depth_,
+                            c1_lr_ ? disp_min_ : -disp_max_, disp_step_),
+        "Stereo::fuseQuadFitDirect");
+  handle_->synchronize();
+}
+
+void Stereo::performLeftRightCheck(Tensor* out, Tensor* left, Tensor* right, float th) {
+  if (!out || !left || !right) arg_fail("Stereo::performLeftRightCheck: NULL tensor");
+  const TensorDesc& d = left->desc();
+  if (!(right->desc() == d) || !(out->desc() == d) || d.c != 1 || d.precision != Precision::FP32)
+    arg_fail("Stereo::performLeftRightCheck: three H x W float32 maps of one size");
+  check(ccs_left_right_check(handle_->get(), left->data(), right->data(), d.h, d.w, th, out->data()),
+        "Stereo::performLeftRightCheck");
+  handle_->synchronize();
 }
 
 void Stereo::computeArgMax(bool lr) {

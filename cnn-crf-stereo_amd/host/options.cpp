@@ -26,7 +26,7 @@ const Spec kSpecs[] = {
     {"matching", "CENSUS", "Matching method: COLORCNN, GRAYCNN, COLORPAIR, LOAD (CENSUS: not implemented)"},
     {"volume", "", "Volume file (.npy, H x W x D) when matching is LOAD"},
     {"inference", "CRF", "Inference method: CRF, ARGMAX, NONE"},
-    {"refinement", "NONE", "Continuous refinement: NONE"},
+    {"refinement", "NONE", "Continuous refinement: NONE, QuadDirect (THuberQuad: not implemented)"},
     {"alpha", "20.0", "Parameter for edge image calculation"},
     {"beta", "0.8", "Parameter for edge image calculation"},
     {"gpu", "0", "GPU to use"},

@@ -64,9 +64,12 @@ static int run(int argc, char** argv) {
   const std::string inference = o.str("inference");
   const auto t0 = std::chrono::steady_clock::now();
   if (inference == "CRF") {
-    if (o.str("refinement") != "NONE") throw Error(CCS_ERR_UNSUPPORTED, "refinement " + o.str("refinement"));
+    const std::string refinement = o.str("refinement");
+    if (refinement != "NONE" && refinement != "QuadDirect")
+      throw Error(CCS_ERR_UNSUPPORTED, "refinement " + refinement);
     stereo.computeSlackPropagation((float)o.num("L1"), (float)o.num("L2"), (float)o.num("delta"),
                                    o.integer("crf-iterations"), alpha, beta, lambda, true);
+    if (refinement == "QuadDirect") stereo.fuseQuadFitDirect();
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     // As the reference saves them: recomputed with lambda = 1 when alpha > 0 (stereo.cpp:425-437).
     std::vector<float> d = to_host(stereo.getOutput()), wx = to_host(stereo.getWx(alpha, beta, true)),

@@ -271,6 +271,22 @@ int ccs_nchw_to_nhwc(ccs_handle_t h, const float* in, int C, int cs, int H, int 
 int ccs_argmin(ccs_handle_t h, const float* cost, int H, int W, int D, float disp_min,
                float disp_step, float* out);
 
+/* QuadDirect sub-pixel refinement, in place on the H x W disparity map
+ * (Stereo::fuseQuadFitDirect, stereo/stereo.cpp:406-409 and
+ * stereo/stereo.cu:924-942,1172-1195): z = (int)((d - disp_min) / disp_step);
+ * for 0 < z < D - 1, d += g / k with g = (c[z-1] - c[z+1]) / (2 step) and
+ * k = max(1e-3, (c[z-1] + c[z+1] - 2 c[z]) / step^2), c the pixel's D volume
+ * values (H x W x D); other pixels keep d. */
+int ccs_quad_fit_direct(ccs_handle_t h, float* disparity, const float* volume, int H, int W, int D,
+                        float disp_min, float disp_step);
+
+/* Left-right check (Stereo::performLeftRightCheck, stereo/stereo.cpp:448-455 and
+ * stereo/stereo.cu:803-817,1198-1225): out = left where |left + R(x - left)| <= th,
+ * else -1; right holds the right-to-left disparities (negative), sampled with
+ * linear interpolation along the row, clamped at the borders. */
+int ccs_left_right_check(ccs_handle_t h, const float* left, const float* right, int H, int W, float th,
+                         float* out);
+
 /* label * disp_step + disp_min (stereo/stereo.cu:841-850, 1229-1246). */
 int ccs_remap(ccs_handle_t h, const int* labels, int H, int W, float disp_min, float disp_step,
               float* out);

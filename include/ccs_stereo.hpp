@@ -403,6 +403,14 @@ class Stereo {
   void computeSlackPropagation(float C1, float C2, float delta, int iterations, float alpha, float beta,
                                float lambda_reg, bool lr);
   void computeArgMax(bool lr);
+  // QuadDirect sub-pixel refinement of the CRF output (stereo/stereo.cpp:406-409):
+  // a parabola through the three values around each label of the final row
+  // unaries of the last computeSlackPropagation (the solver's get_C1 here).
+  void fuseQuadFitDirect();
+  // stereo/stereo.cpp:448-455: out = left where |left + right(x - left)| <= th,
+  // else -1 (right: the right-to-left disparities, negative).
+  void performLeftRightCheck(Tensor* out, Tensor* left, Tensor* right, float th);
+  Tensor* getC1() { return c1_; }  // H x W x D final row unaries of the last CRF run
   void computeVolume(bool lr);
   void computeWeights(float alpha, float beta, float lambda, bool lr);
   // stereo/stereo.cpp:425-437: with alpha > 0 the edge weights are recomputed
@@ -450,6 +458,8 @@ class Stereo {
   Tensor* costvolume_ = nullptr;
   Tensor *pairwise_ = nullptr, *wx_ = nullptr, *wy_ = nullptr;
   Tensor *out_ = nullptr, *labels_ = nullptr;
+  Tensor* c1_ = nullptr;  // final row unaries of the last CRF run (allocated on first use)
+  bool c1_valid_ = false, c1_lr_ = true;
   Tensor *pad1_ = nullptr, *pad2_ = nullptr;
   StereoNet* stereoNet_ = nullptr;
 };

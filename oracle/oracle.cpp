@@ -304,6 +304,39 @@ extern "C" void or_remap(const int* labels, long HW, double disp_min, double dis
   for (long p = 0; p < HW; ++p) out[p] = (double)labels[p] * disp_step + disp_min;
 }
 
+// QuadDirect refinement, stereo/stereo.cu:924-942 (do_fitting_kernel_volume,
+// lambda = 1) and :944-953 (do_argmin_kernel_volume): z = (int)((d - min) /
+// step); for 0 < z < D - 1, d += g / k with g = (c[z-1] - c[z+1]) / (2 step)
+// and k = max(1e-3, (c[z-1] + c[z+1] - 2 c[z]) / step^2).  The reference skips
+// z = 0 and reads past the pixel at z = D - 1; both keep d here.
+extern "C" void or_quad_fit(double* disp, const double* vol, long HW, int D, double disp_min, double disp_step) {
+  for (long p = 0; p < HW; ++p) {
+    const int z = (int)((disp[p] - disp_min) / disp_step);
+    if (z <= 0 || z >= D - 1) continue;
+    const double* c = vol + p * D + z;
+    const double g = (c[-1] - c[1]) / (2.0 * disp_step);
+    const double k = std::max(1e-3, (c[-1] + c[1] - 2.0 * c[0]) / (disp_step * disp_step));
+    disp[p] += g / k;
+  }
+}
+
+// Left-right check, stereo/stereo.cu:803-817 with the clamped linear texture of
+// :84-104: R sampled at texel coordinate x + 0.5 - I1 (linear weights, borders
+// clamped); out = I1 where |I1 + R| <= th, else -1.
+extern "C" void or_lr_check(const double* left, const double* right, int H, int W, double th, double* out) {
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const double I1 = left[(long)y * W + x];
+      const double u = x - I1;  // (x + 0.5 - I1) - 0.5
+      const double fi = std::floor(u);
+      const double a = u - fi;
+      const int i0 = (int)std::min(std::max(fi, 0.0), (double)(W - 1));
+      const int i1 = (int)std::min(std::max(fi + 1.0, 0.0), (double)(W - 1));
+      const double I2 = -((1.0 - a) * right[(long)y * W + i0] + a * right[(long)y * W + i1]);
+      out[(long)y * W + x] = std::fabs(I1 - I2) <= th ? I1 : -1.0;
+    }
+}
+
 extern "C" double or_rho(int d, double L1, double L2, int delta) {
   const int a = d < 0 ? -d : d;
   if (a == 0) return 0.0;

@@ -79,6 +79,12 @@ void or_argmin(const double* vol, long HW, int D, int* idx);
 /* label -> disparity: stereo/stereo.cu:841-850. */
 void or_remap(const int* labels, long HW, double disp_min, double disp_step, double* out);
 
+/* QuadDirect sub-pixel refinement of disp in place (stereo/stereo.cu:924-953). */
+void or_quad_fit(double* disp, const double* vol, long HW, int D, double disp_min, double disp_step);
+
+/* Left-right consistency check (stereo/stereo.cu:803-817). */
+void or_lr_check(const double* left, const double* right, int H, int W, double th, double* out);
+
 /* Pairwise penalty rho of the CRF (0 / L1 for 0<|d|<delta / L2 for |d|>=delta). */
 double or_rho(int d, double L1, double L2, int delta);
 

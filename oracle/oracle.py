@@ -46,6 +46,8 @@ def lib():
             "or_edge_weights": (None, [_D, c_int, c_int, c_double, c_double, c_double, c_double, _D, _D]),
             "or_argmin": (None, [_D, c_long, c_int, _I]),
             "or_remap": (None, [_I, c_long, c_double, c_double, _D]),
+            "or_quad_fit": (None, [_D, _D, c_long, c_int, c_double, c_double]),
+            "or_lr_check": (None, [_D, _D, c_int, c_int, c_double, _D]),
             "or_rho": (c_double, [c_int, c_double, c_double, c_int]),
             "or_energy": (c_double, [_D, _D, _D, c_int, c_int, c_int, _I, c_double, c_double, c_int]),
             "or_chain_viterbi": (c_double, [_D, _D, c_int, c_int, c_double, c_double, c_int, _I]),
@@ -196,6 +198,21 @@ def remap(labels, disp_min, disp_step):
     lab = _i(labels)
     out = np.empty(lab.shape)
     lib().or_remap(_pi(lab), lab.size, disp_min, disp_step, _pd(out))
+    return out
+
+
+def quad_fit(disp, vol, disp_min, disp_step):
+    """QuadDirect refinement: a refined copy of disp (H x W) from vol (H x W x D)."""
+    d = _d(disp).copy()
+    v = _d(vol)
+    lib().or_quad_fit(_pd(d), _pd(v), d.size, v.shape[-1], disp_min, disp_step)
+    return d
+
+
+def lr_check(left, right, th):
+    l, r = _d(left), _d(right)
+    out = np.empty_like(l)
+    lib().or_lr_check(_pd(l), _pd(r), l.shape[0], l.shape[1], th, _pd(out))
     return out
 
 

@@ -232,3 +232,32 @@ def test_cli_with_xy(case, tmp_path, matching, layers_n):
         wxy = pairwise_oracle(case["imgs"][0], pw, pad, (layers_n - 3) // 2)
         np.testing.assert_allclose(np.load(str(out) + "_wx.npy"), wxy[0], atol=1e-4, rtol=0)
         np.testing.assert_allclose(np.load(str(out) + "_wy.npy"), wxy[1], atol=1e-4, rtol=0)
+
+
+def test_cli_quad_direct(case):
+    """--refinement QuadDirect: the CRF labels refined by the parabola through the
+    final row unaries, against the oracle's refinement of its own unaries where
+    the labels agree and the fit is well conditioned (curvature > 0.05)."""
+    t = case["tmp"]
+    base = ["--inference", "CRF", "--L1", "0.5", "--L2", "2", "--delta", "2", "--crf-iterations", "3",
+            "--alpha", "10", "--beta", "1", "--lambda", "1"]
+    hdr = b"Pf\n%d %d\n-1.0\n" % (W, H)
+    read = lambda p: np.frombuffer(open(str(p) + ".pfm", "rb").read()[len(hdr):], dtype="<f4").reshape(H, W)[::-1]
+    run_cli(case, *base, "-o", t / "q0")
+    run_cli(case, *base, "--refinement", "QuadDirect", "-o", t / "q1")
+    d0, d1 = read(t / "q0"), read(t / "q1")
+    run_cli(case, "--inference", "NONE", "--alpha", "10", "--beta", "1", "--lambda", "1", "-o", t / "qv")
+    vol = np.load(str(t / "qv") + ".npy").astype(np.float64)
+    wx, wy = O.edge_weights(case["gray"], 0.7, 10.0, 1.0, 1.0)
+    r = O.slackprop(vol, wx.astype(np.float32), wy.astype(np.float32), 0.5, 2.0, 2, 3, want_A=True)
+    want = O.quad_fit(d0.astype(np.float64), r["A"], 0.0, 1.0)
+    lab = d0.astype(np.int64)
+    A = r["A"]
+    z = np.clip(lab, 1, D - 2)
+    rows, cols = np.indices((H, W))
+    curv = A[rows, cols, z - 1] + A[rows, cols, z + 1] - 2 * A[rows, cols, z]
+    ok = (lab == r["labels"]) & (lab > 0) & (lab < D - 1) & (curv > 0.05)
+    assert ok.sum() > H * W // 4
+    np.testing.assert_allclose(d1[ok], want[ok], rtol=0, atol=2e-3)
+    edge = (lab == 0) | (lab == D - 1)
+    np.testing.assert_array_equal(d1[edge], d0[edge])

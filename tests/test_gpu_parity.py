@@ -322,3 +322,50 @@ def test_ssvm_grad(handle):
     np.testing.assert_allclose(dL.cpu().numpy(), rdL, rtol=1e-12, atol=1e-9)
     np.testing.assert_array_equal(dwx.cpu().numpy(), rdwx)
     np.testing.assert_array_equal(dwy.cpu().numpy(), rdwy)
+
+
+# ---------------------------------------------------------- refinement, LR check
+
+def test_quad_fit_direct(handle):
+    rng = np.random.default_rng(71)
+    H, W, D, dmin, step = 9, 23, 32, 1.0, 0.5
+    z = np.arange(D)[None, None, :]
+    lab = rng.integers(0, D, (H, W))
+    lab[0, :3] = [0, D - 1, D - 2]  # both ends and the last refinable label
+    # a noisy parabola whose vertex lies within half a label of the pixel's label,
+    # so the three values the fit reads are O(1) (well conditioned in float32)
+    centre = lab[..., None] + rng.uniform(-0.5, 0.5, (H, W, 1))
+    vol = (rng.uniform(0.2, 2.0, (H, W, 1)) * (z - centre) ** 2 + rng.normal(0, 0.05, (H, W, D))).astype(np.float32)
+    disp = (dmin + step * lab).astype(np.float32)
+    d = torch.as_tensor(disp, device="cuda").clone()
+    v = torch.as_tensor(vol, device="cuda")
+    capi.call("ccs_quad_fit_direct", handle.h, capi.ptr(d), capi.ptr(v), H, W, D, dmin, step)
+    want = O.quad_fit(disp.astype(np.float64), vol.astype(np.float64), dmin, step)
+    got = d.cpu().numpy()
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
+    assert got[0, 0] == disp[0, 0] and got[0, 1] == disp[0, 1]  # z = 0 and z = D - 1 keep d
+    assert got[0, 2] != disp[0, 2]
+
+
+def test_left_right_check(handle):
+    rng = np.random.default_rng(72)
+    H, W, th = 11, 57, 1.0
+    left = rng.uniform(0, 20, (H, W)).astype(np.float32)
+    right = (-rng.uniform(0, 20, (H, W))).astype(np.float32)
+    right[:, 10:30] = -left[:, 10:30]  # a consistent band (up to the interpolation)
+    out = torch.empty(H, W, device="cuda")
+    lt, rt = torch.as_tensor(left, device="cuda"), torch.as_tensor(right, device="cuda")
+    capi.call("ccs_left_right_check", handle.h, capi.ptr(lt), capi.ptr(rt), H, W, th, capi.ptr(out))
+    got = out.cpu().numpy()
+    want = O.lr_check(left.astype(np.float64), right.astype(np.float64), th)
+    # the float32 kernel and the float64 oracle may disagree only at |I1 - I2| within 1e-4 of th
+    xs = np.arange(W)[None, :] - left.astype(np.float64)
+    i0 = np.clip(np.floor(xs), 0, W - 1).astype(int)
+    i1 = np.clip(np.floor(xs) + 1, 0, W - 1).astype(int)
+    a = xs - np.floor(xs)
+    rows = np.arange(H)[:, None]
+    i2 = -((1 - a) * right[rows, i0] + a * right[rows, i1])
+    clear = np.abs(np.abs(left - i2) - th) > 1e-4
+    np.testing.assert_array_equal(got[clear], want[clear])
+    assert np.all((got == left) | (got == -1.0))
+    assert (got != -1.0).sum() > 0 and (got == -1.0).sum() > 0

@@ -226,6 +226,45 @@ def test_remap_closed_form():
     np.testing.assert_allclose(O.remap(lab, 3.0, 0.5), lab * 0.5 + 3.0, rtol=0)
 
 
+def test_quad_fit_recovers_parabola_vertex():
+    """A volume that is an exact parabola around a sub-label disparity d* is
+    refined to d* (the 3-point fit of a quadratic is the quadratic); labels at
+    the ends and flat volumes stay put."""
+    H, W, D, dmin, step = 4, 5, 16, 2.0, 0.5
+    z0 = RNG.integers(0, D, (H, W))
+    f = RNG.uniform(-0.5, 0.5, (H, W))
+    a = RNG.uniform(0.5, 3.0, (H, W))
+    z = np.arange(D)[None, None, :]
+    vol = a[..., None] * (z - z0[..., None] - f[..., None]) ** 2 + 7.0
+    disp = dmin + step * z0
+    got = O.quad_fit(disp, vol, dmin, step)
+    inner = (z0 > 0) & (z0 < D - 1)
+    np.testing.assert_allclose(got[inner], (dmin + step * (z0 + f))[inner], rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(got[~inner], disp[~inner])
+    np.testing.assert_array_equal(O.quad_fit(disp, np.ones((H, W, D)), dmin, step), disp)
+
+
+def test_lr_check_closed_forms():
+    """Consistent maps pass, maps off by more than th are rejected, a linear
+    right map is interpolated exactly, and samples left of the image clamp to
+    column 0."""
+    H, W = 3, 12
+    left = np.full((H, W), 3.0)
+    np.testing.assert_array_equal(O.lr_check(left, -left, 0.5), left)
+    np.testing.assert_array_equal(O.lr_check(left, -(left + 2.0), 1.0), -np.ones((H, W)))
+    xs = np.arange(W, dtype=np.float64)
+    right = -np.broadcast_to(0.1 * xs + 1.0, (H, W))          # R(x) = -(0.1 x + 1)
+    left = np.broadcast_to(0.25 + 0.3 * xs, (H, W)).copy()    # sample points x - I1 inside the row for x >= 1
+    out = O.lr_check(left, right, 10.0)
+    u = xs - left[0]
+    want_i2 = 0.1 * np.clip(u, 0, W - 1) + 1.0                # clamped linear interpolation of a linear map
+    th = np.abs(left[0] - want_i2)
+    for t in (0.2, 0.6, 1.5):
+        got = O.lr_check(left, right, t)[0]
+        np.testing.assert_array_equal(got, np.where(th <= t, left[0], -1.0))
+    assert np.all(out == left)
+
+
 # --- CRF: energy, chains, SlackProp (restated; stereo/stereo.cpp:129-162)
 
 def _rho_table(D, L1, L2, delta):
