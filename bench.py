@@ -71,19 +71,23 @@ class Launches:
         return out
 
 
-def run_frame(ops, h, frame, left, right, L, A):
-    """One frame, every kernel launch bracketed by events (when enabled)."""
+def run_frame(ops, h, frame, left, right, L, A, side=None):
+    """One frame, every kernel launch bracketed by events (when enabled).
+
+    side = (handle, stream): the right image's normalisation and tower run on
+    that second stream, concurrently with the left tower (the weights are
+    shared read-only; each stream has its own handle), and the correlation
+    waits for both.  Concurrent towers fill the tail of each other's layers."""
+    import torch
+
     H, W = left.shape[1:]
     D = frame.D
     out = {}
     pad = frame.pad
     esize = 2 if frame.tower.precision == "bf16" else 4
-    xs = []
-    for img in (left, right):
-        xs.append(L.run("normalize", {"bytes": 4.0 * img.numel() * 3}, lambda img=img: ops.pad_normalize(h, img, pad)))
-    feats = []
-    for x in xs:
-        y = x
+
+    def tower(hh, img):
+        y = L.run("normalize", {"bytes": 4.0 * img.numel() * 3}, lambda: ops.pad_normalize(hh, img, pad))
         for li, layer in enumerate(frame.tower.layers):
             if li == 0:
                 Hi, Wi = y.shape[1:]
@@ -93,8 +97,21 @@ def run_frame(ops, h, frame, left, right, L, A):
             flops = 2.0 * layer.K * layer.C * layer.KH * layer.KW * Ho * Wo
             cls = "conv_first" if li == 0 else "conv"
             lay = "nchw" if li == 0 else "nhwc"
-            y = L.run(cls, {"flops": flops}, lambda y=y, layer=layer, lay=lay: layer(h, y, layout=lay))
-        feats.append(y)
+            y = L.run(cls, {"flops": flops}, lambda y=y, layer=layer, lay=lay: layer(hh, y, layout=lay))
+        return y
+
+    if side is None:
+        feats = [tower(h, left), tower(h, right)]
+    else:
+        h2, s2 = side
+        main = torch.cuda.current_stream()
+        s2.wait_stream(main)
+        with torch.cuda.stream(s2):
+            fr = tower(h2, right)
+        fl = tower(h, left)
+        main.wait_stream(s2)
+        fr.record_stream(main)
+        feats = [fl, fr]
     C = frame.C
     cs = feats[0].shape[2]
     corr_bytes = 2.0 * H * W * cs * esize + 4.0 * H * W * D
@@ -119,7 +136,7 @@ def run_frame(ops, h, frame, left, right, L, A):
     return out
 
 
-def time_form(pkg, h, data, spec, params, precision, steps, warmup):
+def time_form(pkg, h, data, spec, params, precision, steps, warmup, side=None):
     import torch
     from cnn_crf_stereo_amd import ops, pipeline
 
@@ -129,13 +146,13 @@ def time_form(pkg, h, data, spec, params, precision, steps, warmup):
     H, W = left.shape[1:]
     A = torch.empty((H, W, spec["D"]), dtype=torch.float32, device="cuda")
     for _ in range(warmup):
-        run_frame(ops, h, frame, left, right, Launches(False), A)
+        run_frame(ops, h, frame, left, right, Launches(False), A, side)
     torch.cuda.synchronize()
     L = Launches(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        out = run_frame(ops, h, frame, left, right, L, A)
+        out = run_frame(ops, h, frame, left, right, L, A, side)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / steps
@@ -146,7 +163,7 @@ def time_form(pkg, h, data, spec, params, precision, steps, warmup):
     for _ in range(max(1, steps // 2)):
         left.copy_(torch.from_numpy(hl))
         right.copy_(torch.from_numpy(hr))
-        run_frame(ops, h, frame, left, right, Launches(False), A)
+        run_frame(ops, h, frame, left, right, Launches(False), A, side)
     torch.cuda.synchronize()
     ms_pcie = (time.perf_counter() - t2) * 1e3 / max(1, steps // 2)
     summary = L.summary()
@@ -280,6 +297,8 @@ def main():
                     help="rows of the CPU-baseline band (0 = the whole frame)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurements")
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
+                    help="2: the right tower runs on a second stream beside the left one")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -303,10 +322,15 @@ def main():
     H, W, D = spec["H"], spec["W"], spec["D"]
     pd_per_frame = float(H * W * D)
 
-    ms_bf16, launches_bf16, out_bf16 = time_form(pkg, h, data, spec, params, "bf16", args.steps, args.warmup)
+    side = None
+    if args.streams == 2:
+        s2 = torch.cuda.Stream()  # default priority
+        side = (capi.Handle(s2), s2)
+    ms_bf16, launches_bf16, out_bf16 = time_form(pkg, h, data, spec, params, "bf16", args.steps, args.warmup, side)
     result_extra = {"pcie_inclusive_ms_per_frame": round(launches_bf16.pop("_pcie_inclusive_ms"), 4)}
     if not args.no_fp32:
-        ms_fp32, launches_fp32, out_fp32 = time_form(pkg, h, data, spec, params, "fp32", args.steps, args.warmup)
+        ms_fp32, launches_fp32, out_fp32 = time_form(pkg, h, data, spec, params, "fp32", args.steps, args.warmup,
+                                                     side)
         launches_fp32.pop("_pcie_inclusive_ms")
         result_extra["fp32_ms_per_frame"] = round(ms_fp32, 4)
         result_extra["fp32_pixel_disp_per_s"] = pd_per_frame / (ms_fp32 / 1e3)
@@ -397,7 +421,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "%s: grayscale random-dot pair %dx%d, D=%d, 7-layer 100-channel CNN (%s), "
                                "%d SlackProp iterations" % (args.workload, W, H, D, args.arch, spec["iters"]),
-                   "form": "BF16-MFMA CNN + correlation (FP32 accumulate), FP32 CRF min-sum, FP64 bound"},
+                   "form": "BF16-MFMA CNN + correlation (FP32 accumulate), FP32 CRF min-sum, FP64 bound",
+                   "streams": "towers on %d HIP stream%s" % (args.streams, "s" if args.streams > 1 else "")},
         "roofline": roof,
         "cpu_baseline": cpu,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},

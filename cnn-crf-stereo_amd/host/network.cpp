@@ -43,8 +43,13 @@ StereoNet::StereoNet(int numLayers, int in, int ic, int ih, int iw)
 }
 
 StereoNet::~StereoNet() {
+  if (m_stream2) (void)hipStreamSynchronize((hipStream_t)m_stream2);
   for (auto* v : {&m_leftOps, &m_rightOps, &m_lrOps, &m_pairwiseOps})
     for (Operator* op : *v) delete op;
+  m_handle2.reset();
+  if (m_evIn) (void)hipEventDestroy((hipEvent_t)m_evIn);
+  if (m_evOut) (void)hipEventDestroy((hipEvent_t)m_evOut);
+  if (m_stream2) (void)hipStreamDestroy((hipStream_t)m_stream2);
 }
 
 void StereoNet::initNet(float min_disp, float max_disp, float step, int rect_corr, Tensor* d_unaryOut,
@@ -154,19 +159,48 @@ void StereoNet::setDisparities(float min_disp, float max_disp, float step, Tenso
 Tensor* StereoNet::performPrediction(Tensor* d_inputLeft, Tensor* d_inputRight) {
   if (!m_initialized) arg_fail("StereoNet: call initNet first");
   Handle& h = *m_handle;
+  // Without garbage collection the two towers run concurrently: the right one
+  // on a second, non-blocking stream of default priority (each layer's tail
+  // then overlaps the other tower's layer).  With it, every free synchronises
+  // the device anyway, so one stream is used.
+  const bool two = !m_allowGc;
+  if (two && !m_stream2) {
+    hipStream_t s2;
+    hipEvent_t e0, e1;
+    if (hipStreamCreateWithFlags(&s2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e0, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e1, hipEventDisableTiming) != hipSuccess)
+      throw Error(CCS_ERR_HIP, "StereoNet: creating the second stream");
+    m_stream2 = s2;
+    m_evIn = e0;
+    m_evOut = e1;
+    m_handle2.reset(new Handle(s2));
+  }
+  auto hip_ok = [](hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(CCS_ERR_HIP, std::string("StereoNet: ") + what + ": " + hipGetErrorString(e));
+  };
+  if (two) {  // the right tower starts after everything queued so far on the main stream
+    hip_ok(hipEventRecord((hipEvent_t)m_evIn, (hipStream_t)h.stream()), "event record");
+    hip_ok(hipStreamWaitEvent((hipStream_t)m_stream2, (hipEvent_t)m_evIn, 0), "stream wait");
+  }
   Tensor* feats[2] = {d_inputLeft, d_inputRight};
-  for (int t = 0; t < 2; ++t) {
+  for (int t = 1; t >= 0; --t) {  // right tower first, so it is queued on its stream before the left one
     std::vector<Operator*>& ops = t == 0 ? m_leftOps : m_rightOps;
+    Handle& ht = (two && t == 1) ? *m_handle2 : h;
     Tensor* x = feats[t];
     Convolution* prev = nullptr;
     for (Operator* op : ops) {
       if (op->type() != Operator::CONVOLUTION) continue;  // bias + tanh run in the convolution epilogue
       auto* conv = static_cast<Convolution*>(op);
-      x = conv->forward(x, h);
+      x = conv->forward(x, ht);
       if (prev && m_allowGc) prev->deleteOutputMemory();  // freed once its consumer is queued
       prev = conv;
     }
     feats[t] = x;
+  }
+  if (two) {  // the correlation waits for the right tower
+    hip_ok(hipEventRecord((hipEvent_t)m_evOut, (hipStream_t)m_stream2), "event record");
+    hip_ok(hipStreamWaitEvent((hipStream_t)h.stream(), (hipEvent_t)m_evOut, 0), "stream wait");
   }
   std::vector<Tensor*> in{feats[0], feats[1]};
   Tensor* vol = m_lrOps[0]->forward(in, h);  // correlation + fused -softmax

@@ -44,6 +44,7 @@ const Spec kSpecs[] = {
     {"ref-iterations", "10", "Refinement iterations"},
     {"warps", "2", "Refinement warps"},
     {"precision", "bf16", "Feature CNN arithmetic on MI355X: bf16 (MFMA) or fp32"},
+    {"cnn-gc", "true", "Free each feature map once consumed (false: keep them and run the two towers on two streams)"},
 };
 
 const Spec* find(const std::string& key) {

@@ -33,7 +33,7 @@ static int run(int argc, char** argv) {
   if (I1.width != I2.width || I1.height != I2.height) throw Error(CCS_ERR_ARG, "the two images differ in size");
 
   Stereo stereo;
-  stereo.setAllowCnnGc(true);
+  stereo.setAllowCnnGc(o.flag("cnn-gc"));
   stereo.setWithXy(o.flag("with-xy"));
   stereo.initialize(I1.width, I1.height, (float)o.num("disp-min"), (float)o.num("disp-step"),
                     (float)o.num("disp-max"));

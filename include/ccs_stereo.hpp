@@ -359,6 +359,12 @@ class StereoNet {
   void runPairwise(Tensor* d_inputLeft);
 
   std::unique_ptr<Handle> m_handle;
+  // The right tower runs on a second stream beside the left one (HIP events
+  // order it after the inputs and the correlation after it).
+  std::unique_ptr<Handle> m_handle2;
+  void* m_stream2 = nullptr;
+  void* m_evIn = nullptr;
+  void* m_evOut = nullptr;
   std::vector<Operator*> m_leftOps, m_rightOps, m_lrOps, m_pairwiseOps;
   Tensor* m_pairwiseOut = nullptr;  // 1 x 2 x H x W: wx, wy (caller-owned)
   int m_pwCrop = 0;                 // pixels cropped from each side of the pairwise net's input

@@ -261,3 +261,15 @@ def test_cli_quad_direct(case):
     np.testing.assert_allclose(d1[ok], want[ok], rtol=0, atol=2e-3)
     edge = (lab == 0) | (lab == D - 1)
     np.testing.assert_array_equal(d1[edge], d0[edge])
+
+
+def test_cli_two_stream_towers_match(case):
+    """--cnn-gc false keeps the feature maps and runs the right tower on a
+    second stream beside the left one; the volume must be bit-identical to the
+    one-stream run."""
+    t = case["tmp"]
+    run_cli(case, "--inference", "NONE", "--alpha", "10", "--beta", "1", "--lambda", "1", "-o", t / "g1")
+    run_cli(case, "--inference", "NONE", "--alpha", "10", "--beta", "1", "--lambda", "1", "--cnn-gc", "false",
+            "-o", t / "g0")
+    np.testing.assert_array_equal(np.load(str(t / "g0") + ".npy"), np.load(str(t / "g1") + ".npy"))
+    np.testing.assert_allclose(np.load(str(t / "g0") + ".npy"), case["cost"], atol=1e-4, rtol=0)
