@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256, MINW) void k_conv_bf16(const bf16* __restrict_
       for (int e = 0; e < 4; ++e) {
         const int px = 16 * m + 4 * lgrp + e;
         float v = acc[m][t][e] + bn;
-        if (act) v = tanh_fast(v);
+        if (act) v = tanh_for_bf16(v);  // the result is rounded to bf16 below
         if (n < Cs) Os[(wave * TW + px) * Cs + n] = f32_to_bf16(n < K ? v : 0.f);
       }
   }
@@ -408,7 +408,7 @@ __global__ void k_conv_generic(const TI* __restrict__ x, int x_nchw, int Hin, in
   for (int e = 0; e < 8; ++e) {
     const int n = g * 8 + e;
     float v = acc[e];
-    if (act) v = tanh_fast(v);
+    if (act) v = sizeof(TO) == 2 ? tanh_for_bf16(v) : tanh_fast(v);
     y[p * Cs_out + n] = (TO)(n < K ? v : 0.f);
   }
 }
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void k_conv_image(const float* __restrict__ x,
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       float v = acc[e];
-      if (act) v = tanh_fast(v);
+      if (act) v = sizeof(TO) == 2 ? tanh_for_bf16(v) : tanh_fast(v);
       os[threadIdx.x * Cs + k0 + e] = (TO)(k0 + e < K ? v : 0.f);
     }
   }
