@@ -161,11 +161,16 @@ __global__ __launch_bounds__(256, MINW) void k_conv_bf16(const bf16* __restrict_
   CCS_B_STORE(0);
   __syncthreads();
 
+  // The accumulators start at the bias (a lane's four values share channel
+  // 16 t + lrow), so the epilogue has no add.
   f32x4 acc[MT][NT];
 #pragma unroll
-  for (int m = 0; m < MT; ++m)
+  for (int t = 0; t < NT; ++t) {
+    const int n = 16 * t + lrow;
+    const float bn = (n < K && bias) ? bias[n] : 0.f;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MT; ++m) acc[m][t] = f32x4{bn, bn, bn, bn};
+  }
 
   auto load_frags = [&](int q, int ql, bf16x8 (&a)[MT], bf16x8 (&b)[NT]) {
     const int kg = 4 * q + lgrp;  // 8-element group of the reduction index
@@ -218,22 +223,28 @@ __global__ __launch_bounds__(256, MINW) void k_conv_bf16(const bf16* __restrict_
     }
   }
 
-  // Epilogue: bias + tanh -> bf16 staged in the input-tile region, then contiguous stores.
+  // Epilogue: tanh -> bf16 staged in the input-tile region, then contiguous
+  // stores.  The activation is chosen once, outside the element loops.  The
+  // padding channels K <= n < Cs come out as exact zeros: their packed
+  // weights and bias are zero, and tanh(0) = 0.
   bf16* Os = As;
+  auto epilogue = [&](auto f) {
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int n = 16 * t + lrow;
-    const float bn = (n < K && bias) ? bias[n] : 0.f;
+    for (int t = 0; t < NT; ++t) {
+      const int n = 16 * t + lrow;
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int px = 16 * m + 4 * lgrp + e;
-        float v = acc[m][t][e] + bn;
-        if (act) v = tanh_for_bf16(v);  // the result is rounded to bf16 below
-        if (n < Cs) Os[(wave * TW + px) * Cs + n] = f32_to_bf16(n < K ? v : 0.f);
-      }
-  }
+        for (int e = 0; e < 4; ++e) {
+          const int px = 16 * m + 4 * lgrp + e;
+          if (n < Cs) Os[(wave * TW + px) * Cs + n] = f32_to_bf16(f(acc[m][t][e]));
+        }
+    }
+  };
+  if (act)
+    epilogue([](float v) { return tanh_for_bf16(v); });  // the result is rounded to bf16
+  else
+    epilogue([](float v) { return v; });
   __syncthreads();
   constexpr int vec_per_row = TW * CG;
   for (int v = tid; v < kTH * vec_per_row; v += 256) {
