@@ -72,6 +72,7 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 // Native 16-byte vector (unlike HIP's uint4 struct it stays in registers when copied).
 typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
 
 __device__ __forceinline__ float bf16_to_f32(bf16 v) { return (float)v; }
 __device__ __forceinline__ bf16 f32_to_bf16(float v) { return (bf16)v; }
@@ -97,13 +98,14 @@ __device__ __forceinline__ float tanh_fast(float x) {
   return copysignf(ax < 0.625f ? p : t, x);
 }
 
-// tanh for results that are stored in bf16 (8-bit mantissa): (1 - e) / (1 + e)
-// with e = exp(-2|x|), sign restored.  Absolute error below 2e-7 everywhere
-// (relative error grows towards 0 as 1e-7 / |x|, far under bf16's 2^-9
-// rounding), two transcendental and five plain instructions, no branch.
+// tanh for results that are stored in bf16 (8-bit mantissa):
+// 2 / (1 + exp(-2x)) - 1.  Absolute error about 2e-7 everywhere (relative
+// error grows towards 0 as 1e-7 / |x|, far under bf16's 2^-9 rounding); the
+// limits are exact (exp -> inf gives -1, exp -> 0 gives 1).  Two
+// transcendental and three plain instructions, no branch, no sign fix-up.
 __device__ __forceinline__ float tanh_for_bf16(float x) {
-  const float e = __builtin_amdgcn_exp2f(fabsf(x) * -2.8853900817779268f);
-  return copysignf((1.f - e) * __builtin_amdgcn_rcpf(1.f + e), x);
+  const float e = __builtin_amdgcn_exp2f(x * -2.8853900817779268f);  // exp(-2x)
+  return fmaf(2.f, __builtin_amdgcn_rcpf(1.f + e), -1.f);
 }
 
 // ---------------------------------------------------------------------------

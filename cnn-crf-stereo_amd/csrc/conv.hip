@@ -161,15 +161,21 @@ __global__ __launch_bounds__(256, MINW) void k_conv_bf16(const bf16* __restrict_
   CCS_B_STORE(0);
   __syncthreads();
 
-  // The accumulators start at the bias (a lane's four values share channel
-  // 16 t + lrow), so the epilogue has no add.
+  // The MFMAs compute the transposed block (weights as the A operand): lane
+  // (lrow, lgrp) accumulates pixel 16 m + lrow, channels 16 t + 4 lgrp .. +3,
+  // so the epilogue stores four consecutive channels per lane at once.  The
+  // accumulators start at the bias, so the epilogue has no add.
   f32x4 acc[MT][NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int n = 16 * t + lrow;
-    const float bn = (n < K && bias) ? bias[n] : 0.f;
+    f32x4 bn;
 #pragma unroll
-    for (int m = 0; m < MT; ++m) acc[m][t] = f32x4{bn, bn, bn, bn};
+    for (int e = 0; e < 4; ++e) {
+      const int n = 16 * t + 4 * lgrp + e;
+      bn[e] = (n < K && bias) ? bias[n] : 0.f;
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[m][t] = bn;
   }
 
   auto load_frags = [&](int q, int ql, bf16x8 (&a)[MT], bf16x8 (&b)[NT]) {
@@ -207,7 +213,7 @@ __global__ __launch_bounds__(256, MINW) void k_conv_bf16(const bf16* __restrict_
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[t], acc[m][t], 0, 0, 0);
+        for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[m], acc[m][t], 0, 0, 0);
 #if CCS_CONV_SCHED
       __builtin_amdgcn_sched_barrier(0);
 #endif
@@ -231,14 +237,15 @@ __global__ __launch_bounds__(256, MINW) void k_conv_bf16(const bf16* __restrict_
   auto epilogue = [&](auto f) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const int n = 16 * t + lrow;
+      const int n0 = 16 * t + 4 * lgrp;  // Cs is a multiple of 8: a group of 4 is all in or all out
 #pragma unroll
-      for (int m = 0; m < MT; ++m)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int px = 16 * m + 4 * lgrp + e;
-          if (n < Cs) Os[(wave * TW + px) * Cs + n] = f32_to_bf16(f(acc[m][t][e]));
-        }
+      for (int m = 0; m < MT; ++m) {
+        const int px = 16 * m + lrow;
+        typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4;
+        const f32x4 r = {f(acc[m][t][0]), f(acc[m][t][1]), f(acc[m][t][2]), f(acc[m][t][3])};
+        const bf16x4 v = __builtin_convertvector(r, bf16x4);  // two v_cvt_pk_bf16_f32
+        if (n0 < Cs) *reinterpret_cast<bf16x4*>(Os + (wave * TW + px) * Cs + n0) = v;
+      }
     }
   };
   if (act)

@@ -256,7 +256,6 @@ __device__ __forceinline__ void st_node(float* __restrict__ base, int lane, int 
 // instructions per node instead of 64-bit address arithmetic.  Every offset
 // passed here is that of a node inside the resource's range.
 typedef __amdgpu_buffer_rsrc_t Rsrc;
-typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
 
 __device__ __forceinline__ Rsrc chain_rsrc(const float* base, long nfloats) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)(nfloats * 4), 0x00020000);
