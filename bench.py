@@ -171,6 +171,39 @@ def time_form(pkg, h, data, spec, params, precision, steps, warmup, side=None):
     return ms, summary, out
 
 
+def time_pipelined(pkg, lanes, data, spec, params, precision, steps, warmup):
+    """Steady-state ms per frame with len(lanes) frames in flight: frame i runs
+    entirely on lane i % len(lanes), a (handle, stream) pair with its own CRF
+    state, so consecutive frames overlap wherever the GPU has room (the
+    latency-bound CRF row sweeps leave most SIMDs idle)."""
+    import torch
+    from cnn_crf_stereo_amd import ops, pipeline
+
+    left = torch.as_tensor(data["left"], device="cuda")
+    right = torch.as_tensor(data["right"], device="cuda")
+    H, W = left.shape[1:]
+    frames = [pipeline.Frame(h, params, precision, spec["D"], iters=spec["iters"]) for (h, _) in lanes]
+    As = [torch.empty((H, W, spec["D"]), dtype=torch.float32, device="cuda") for _ in lanes]
+    main = torch.cuda.current_stream()
+
+    def go(i):
+        h, st = lanes[i % len(lanes)]
+        with torch.cuda.stream(st):
+            return run_frame(ops, h, frames[i % len(lanes)], left, right, Launches(False), As[i % len(lanes)])
+
+    torch.cuda.synchronize()
+    for st in [st for (_, st) in lanes]:
+        st.wait_stream(main)
+    for i in range(warmup * len(lanes)):
+        go(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        out = go(i)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, out
+
+
 def measured_traffic(workload):
     """HBM bytes per launch by kernel class, from the committed counter summary
     (rocprofv3 FETCH_SIZE + WRITE_SIZE passes, scaled by a streaming kernel of
@@ -299,6 +332,8 @@ def main():
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurements")
     ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
                     help="2: the right tower runs on a second stream beside the left one")
+    ap.add_argument("--inflight", type=int, default=2,
+                    help="frames in flight for the pipelined throughput figure (0: skip it)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -328,6 +363,19 @@ def main():
         side = (capi.Handle(s2), s2)
     ms_bf16, launches_bf16, out_bf16 = time_form(pkg, h, data, spec, params, "bf16", args.steps, args.warmup, side)
     result_extra = {"pcie_inclusive_ms_per_frame": round(launches_bf16.pop("_pcie_inclusive_ms"), 4)}
+    if args.inflight > 1:
+        lanes = []
+        for _ in range(args.inflight):
+            st = torch.cuda.Stream()  # default priority
+            lanes.append((capi.Handle(st), st))
+        pipe = {}
+        for prec in ("bf16", "fp32") if not args.no_fp32 else ("bf16",):
+            ms_p, out_p = time_pipelined(pkg, lanes, data, spec, params, prec, args.steps, 1)
+            pipe[prec] = round(ms_p, 4)
+            if prec == "bf16":
+                assert bool((out_p["labels"] == out_bf16["labels"]).all()), "pipelined frames must match"
+        result_extra["pipelined"] = {"frames_in_flight": args.inflight, "ms_per_frame": pipe,
+                                     "pixel_disp_per_s": {k: pd_per_frame / (v / 1e3) for k, v in pipe.items()}}
     if not args.no_fp32:
         ms_fp32, launches_fp32, out_fp32 = time_form(pkg, h, data, spec, params, "fp32", args.steps, args.warmup,
                                                      side)
