@@ -21,7 +21,8 @@
 // GPU mapping: one wavefront per chain, label s = lane * LPL + j held in VGPRs,
 // DPP wave_shr/wave_shl for the label neighbours and a DPP tree for the minimum
 // over labels.  The volume streams through HBM once per pass (node i is D
-// contiguous floats), backward messages go to a scratch volume, and the
+// contiguous floats), backward messages go to a scratch volume (column sweeps
+// of large volumes store every seg_len()-th one and recompute the rest), and the
 // per-node minima go to LDS; their float64 sums are formed by wave scans after
 // each pass.  The row decode runs its two message passes on two wavefronts at
 // once (k_decode2).
@@ -47,48 +48,35 @@ constexpr float kTieEps = CCS_CRF_TIE_EPS;
 // fill the DPP/permlane wait states of the neighbouring step).
 constexpr int kSchedValu = 0x2;
 
+// kNoWideStores: no chain kernel stores more than 64 bits per lane in one
+// instruction.  A vector store of 96 or 128 bits reads its data VGPRs after
+// issue, and hipcc (ROCm 7.2) for gfx950 was seen to place a VALU write of those
+// VGPRs (a DPP min of the next step) directly behind such a store with no wait
+// state; the D = 256 sweeps then stored partly overwritten data, varying from
+// run to run.  With two 64-bit stores (kept apart by a compiler barrier, which
+// stops them being merged back) the sweeps are deterministic and bit-identical
+// between the column schemes.
+
+// Column sweeps of large volumes keep the backward messages of every
+// seg_len()-th node only; the forward pass recomputes a segment's messages from
+// the checkpoint at its end, bit for bit (0: every message is stored and
+// re-read).
+#ifndef CCS_CRF_SEG
+#define CCS_CRF_SEG 12
+#endif
+constexpr int kSeg = CCS_CRF_SEG;
+// Segment length by labels per lane: a segment keeps two segments' unaries and
+// costs and one segment's messages in registers, so four labels per lane take
+// half the length (two waves per SIMD without spills).
+template <int LPL>
+constexpr int seg_len() {
+  return kSeg <= 0 ? 0 : LPL >= 4 ? (kSeg / 2 > 0 ? kSeg / 2 : 1) : kSeg;
+}
+
 template <int LPL>
 struct Vec {
   float v[LPL];
 };
-
-template <int LPL>
-__device__ __forceinline__ void load_node(const float* __restrict__ base, int lane, int D, Vec<LPL>& out,
-                                          float fill) {
-  const int s0 = lane * LPL;
-  if (D % LPL == 0 && s0 + LPL <= D) {  // 8/16-byte aligned: node offsets are multiples of D
-    if constexpr (LPL == 4) {
-      const float4 t = *reinterpret_cast<const float4*>(base + s0);
-      out.v[0] = t.x; out.v[1] = t.y; out.v[2] = t.z; out.v[3] = t.w;
-    } else if constexpr (LPL == 2) {
-      const float2 t = *reinterpret_cast<const float2*>(base + s0);
-      out.v[0] = t.x; out.v[1] = t.y;
-    } else {
-      out.v[0] = base[s0];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < LPL; ++j) out.v[j] = (s0 + j < D) ? base[s0 + j] : fill;
-  }
-}
-
-template <int LPL>
-__device__ __forceinline__ void store_node(float* __restrict__ base, int lane, int D, const Vec<LPL>& x) {
-  const int s0 = lane * LPL;
-  if (D % LPL == 0 && s0 + LPL <= D) {
-    if constexpr (LPL == 4) {
-      *reinterpret_cast<float4*>(base + s0) = make_float4(x.v[0], x.v[1], x.v[2], x.v[3]);
-    } else if constexpr (LPL == 2) {
-      *reinterpret_cast<float2*>(base + s0) = make_float2(x.v[0], x.v[1]);
-    } else {
-      base[s0] = x.v[0];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < LPL; ++j)
-      if (s0 + j < D) base[s0 + j] = x.v[j];
-  }
-}
 
 // Vector shifted by one label: out(s) = x(s - 1) (dir < 0) or x(s + 1) (dir > 0).
 template <int LPL>
@@ -236,8 +224,10 @@ __device__ __forceinline__ Vec<LPL> ld_node(const float* __restrict__ base, int 
 template <int LPL, bool FULL>
 __device__ __forceinline__ void st_node(float* __restrict__ base, int lane, int D, const Vec<LPL>& x) {
   if constexpr (FULL) {
-    if constexpr (LPL == 4) {
-      *reinterpret_cast<float4*>(base + lane * 4) = make_float4(x.v[0], x.v[1], x.v[2], x.v[3]);
+    if constexpr (LPL == 4) {  // two 64-bit stores (see kNoWideStores)
+      *reinterpret_cast<float2*>(base + lane * 4) = make_float2(x.v[0], x.v[1]);
+      asm volatile("" ::: "memory");
+      *reinterpret_cast<float2*>(base + lane * 4 + 2) = make_float2(x.v[2], x.v[3]);
     } else if constexpr (LPL == 2) {
       *reinterpret_cast<float2*>(base + lane * 2) = make_float2(x.v[0], x.v[1]);
     } else {
@@ -287,10 +277,12 @@ __device__ __forceinline__ Vec<LPL> ld_buf(Rsrc r, int soff, int lane, int D, fl
 
 template <int LPL, bool FULL>
 __device__ __forceinline__ void st_buf(Rsrc r, int soff, int lane, int D, const Vec<LPL>& x) {
-  if constexpr (FULL && LPL == 4) {
-    const u32x4 t = {__float_as_uint(x.v[0]), __float_as_uint(x.v[1]), __float_as_uint(x.v[2]),
-                     __float_as_uint(x.v[3])};
-    __builtin_amdgcn_raw_buffer_store_b128(t, r, lane * 16, soff, 0);
+  if constexpr (FULL && LPL == 4) {  // two 64-bit stores (see kNoWideStores)
+    const u32x2 t = {__float_as_uint(x.v[0]), __float_as_uint(x.v[1])};
+    const u32x2 u = {__float_as_uint(x.v[2]), __float_as_uint(x.v[3])};
+    __builtin_amdgcn_raw_buffer_store_b64(t, r, lane * 16, soff, 0);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_raw_buffer_store_b64(u, r, lane * 16 + 8, soff, 0);
   } else if constexpr (FULL && LPL == 2) {
     const u32x2 t = {__float_as_uint(x.v[0]), __float_as_uint(x.v[1])};
     __builtin_amdgcn_raw_buffer_store_b64(t, r, lane * 8, soff, 0);
@@ -373,7 +365,7 @@ __device__ double chain_kept(const float* v, const double* b, int n, int lane, d
 // of registers and the load for node s + P is issued right after slot s is read
 // (clamped to the last node, so it never leaves the chain).  The last partial
 // block is an unrolled tail that issues no loads.
-template <int LPL, int MODE, int P, bool FULL, int DC>
+template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG>
 __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
   constexpr bool kCols = MODE == kColsMinorant;
@@ -444,8 +436,12 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, a.L1, a.L2, a.delta);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) bh.v[k] = valid[k] ? m.v[k] - u : 0.f;
-    if constexpr (kCols) st_node<LPL, FULL>(msg + (long)i * D, lane, D, bh);
-    else st_buf<LPL, FULL>(rM, i * D * 4, lane, D, bh);
+    if constexpr (kCols) {
+      constexpr int K = SEG ? seg_len<LPL>() : 0;
+      if (K == 0 || i % K == 0) st_node<LPL, FULL>(msg + (long)i * D, lane, D, bh);  // i is wave-uniform
+    } else {
+      st_buf<LPL, FULL>(rM, i * D * 4, lane, D, bh);
+    }
     u_lds[i] = u;  // every lane writes the same value
   };
   if (nb > 0) {
@@ -503,6 +499,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     // The float64 kept constant (the chain minimum this pass leaves behind) is
     // formed from the per-node minima v_i after the pass: chain_kept().
     // step_f takes RB = R + Bh_i, A_{i+1} and theta_{i+1}.
+    // step_f returns the node's new unaries; the caller stores them.
     auto step_f = [&](int i, const Vec<LPL>& RB, const Vec<LPL>& an, const Vec<LPL>& thn, float wi) {
       Vec<LPL> gt, V;
 #pragma unroll
@@ -526,10 +523,90 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         out.v[k] = MODE == kRowsMinorant ? a_cur.v[k] - g : a_cur.v[k] + g;
         R.v[k] = valid[k] ? thn.v[k] + m.v[k] - v : kInfF;
       }
+      a_cur = an;
+      return out;
+    };
+    auto store_f = [&](int i, const Vec<LPL>& out) {
       if constexpr (kCols) st_node<LPL, FULL>(Aw + i * nstride, lane, D, out);
       else st_buf<LPL, FULL>(rA, i * strideB, lane, D, out);
-      a_cur = an;
     };
+    if constexpr (kCols && SEG && seg_len<LPL>() > 0) {
+      // Segments of K forward steps i0 .. i0+K-1.  Their nodes i0+1 .. i0+K
+      // (unaries and costs), the checkpoint Bh_{min(i0+K, n-1)} (Bh_{n-1} = 0
+      // is stored) and the segment's minima u_i and weights w_i (one per lane)
+      // are loaded a segment ahead.  Bh_i for the segment is recomputed
+      // backwards from the checkpoint with the backward step's own operations
+      // and operands (theta_{i+1}, u_i, w_i), so it equals the message the
+      // backward pass formed; u_lds[i] still holds the backward minimum until
+      // forward step i overwrites it.
+      constexpr int K = seg_len<LPL>();
+      struct Seg {
+        Vec<LPL> a[K], c[K], m;
+        float u, w;
+      };
+      Seg X, Y;
+      Vec<LPL> bs[K];
+      // Nodes first .. first+K-1 (clamped to the chain) through buffer
+      // resources based at the segment: 32-bit offsets, one scalar multiply
+      // per node (the host bounds K nodes of a column by 2 GiB).
+      auto load_seg = [&](int i0, Seg& B) {
+        const int first = min(i0 + 1, n - 1);
+        const long span = (long)(K - 1) * nstride + D;
+        const Rsrc sA = chain_rsrc(Ab + first * nstride, span);
+        const Rsrc sC = chain_rsrc(Cb + first * nstride, span);
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+          const int off = min(t, n - 1 - first) * strideB;
+          B.a[t] = ld_buf<LPL, FULL>(sA, off, lane, D, kInfF);
+          B.c[t] = ld_buf<LPL, FULL>(sC, off, lane, D, kInfF);
+        }
+        B.m = ld_node<LPL, FULL>(msg + (long)min(i0 + K, nb) * D, lane, D, 0.f);
+        const int e = max(min(i0 + lane, n - 2), 0);
+        B.u = u_lds[e];
+        B.w = w_lds[e];
+      };
+      // The segment of cnt (1 .. K) steps from i0, its data in B.
+      auto run_seg = [&](int i0, int cnt, const Seg& B) {
+        Vec<LPL> Xb = B.m;
+#pragma unroll
+        for (int t = K - 1; t >= 0; --t) {
+          if (t < cnt) {
+            const Vec<LPL> U = plus(theta_of(B.a[t], B.c[t]), Xb);
+            const float u = lane_weight(B.u, t);
+            const Vec<LPL> m = minconv<LPL, DC>(U, u, lane_weight(B.w, t), a.L1, a.L2, a.delta);
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) Xb.v[k] = valid[k] ? m.v[k] - u : 0.f;
+            bs[t] = Xb;
+          }
+        }
+        const Rsrc sW = chain_rsrc(Aw + i0 * nstride, (long)(K - 1) * nstride + D);
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+          if (t < cnt)
+            st_buf<LPL, FULL>(sW, t * strideB, lane, D,
+                              step_f(i0 + t, plus(R, bs[t]), B.a[t], theta_of(B.a[t], B.c[t]), lane_weight(B.w, t)));
+      };
+      // Two segments per iteration, alternating between X and Y; the next
+      // segment's loads are issued before the current segment's steps.
+      load_seg(0, X);
+      int i0 = 0;
+      for (;;) {
+        if (i0 + K >= nb) break;  // X holds the last segment
+        load_seg(i0 + K, Y);
+        __builtin_amdgcn_sched_barrier(0);
+        run_seg(i0, K, X);
+        i0 += K;
+        if (i0 + K >= nb) {
+          X = Y;
+          break;
+        }
+        load_seg(i0 + K, X);
+        __builtin_amdgcn_sched_barrier(0);
+        run_seg(i0, K, Y);
+        i0 += K;
+      }
+      if (i0 < nb) run_seg(i0, nb - i0, X);
+    } else {
     if (nb > 0) {
 #pragma unroll
       for (int j = 0; j < P; ++j) load_f(j, rb[j], ra[j], rc[j]);
@@ -550,13 +627,14 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         }
         __builtin_amdgcn_sched_barrier(kSchedValu);
         load_f(i0 + j + P, rb[j], ra[j], rc[j]);
-        step_f(i0 + j, RB, an, thn, lane_weight(wf, j));
+        store_f(i0 + j, step_f(i0 + j, RB, an, thn, lane_weight(wf, j)));
       }
       wf = wnext;
     }
 #pragma unroll
     for (int j = 0; j < P; ++j)
-      if (i0 + j < nb) step_f(i0 + j, plus(R, rb[j]), ra[j], theta_of(ra[j], rc[j]), lane_weight(wf, j));
+      if (i0 + j < nb) store_f(i0 + j, step_f(i0 + j, plus(R, rb[j]), ra[j], theta_of(ra[j], rc[j]), lane_weight(wf, j)));
+    }
     {
       Vec<LPL> out;
 #pragma unroll
@@ -760,31 +838,40 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   }
 }
 
+template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG>
+static void launch_sweep(ChainArgs& a, size_t lds, hipStream_t st) {
+  if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
+    (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC, SEG>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  k_chain<LPL, MODE, P, FULL, DC, SEG><<<a.n_chains, 64, lds, st>>>(a);
+}
+
 template <int LPL, int MODE, int P, bool FULL, int DC>
-static void launch_one(ChainArgs& a, size_t lds, hipStream_t st) {
-  if constexpr (MODE != kRowsDecode)
-    if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
-      (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+static void launch_one(ChainArgs& a, size_t lds, bool seg, hipStream_t st) {
   if constexpr (MODE == kRowsDecode) {
     const size_t lds2 = 2 * sizeof(float) * (size_t)a.n;
     if (a.gap)
       k_decode2<LPL, P, FULL, DC, true><<<a.n_chains, 128, lds2, st>>>(a);
     else
       k_decode2<LPL, P, FULL, DC, false><<<a.n_chains, 128, lds2, st>>>(a);
+  } else if constexpr (MODE == kColsMinorant) {
+    if (seg)
+      launch_sweep<LPL, MODE, P, FULL, DC, true>(a, lds, st);
+    else
+      launch_sweep<LPL, MODE, P, FULL, DC, false>(a, lds, st);
   } else {
-    k_chain<LPL, MODE, P, FULL, DC><<<a.n_chains, 64, lds, st>>>(a);
+    launch_sweep<LPL, MODE, P, FULL, DC, false>(a, lds, st);
   }
 }
 
 template <int MODE, int LPL, int P>
-static void launch_lpl(ChainArgs& a, size_t lds, hipStream_t st) {
+static void launch_lpl(ChainArgs& a, size_t lds, bool seg, hipStream_t st) {
   if (a.D == 64 * LPL && a.delta == 2)
-    launch_one<LPL, MODE, P, true, 2>(a, lds, st);
+    launch_one<LPL, MODE, P, true, 2>(a, lds, seg, st);
   else if (a.D == 64 * LPL)
-    launch_one<LPL, MODE, P, true, 0>(a, lds, st);
+    launch_one<LPL, MODE, P, true, 0>(a, lds, seg, st);
   else
-    launch_one<LPL, MODE, P, false, 0>(a, lds, st);
+    launch_one<LPL, MODE, P, false, 0>(a, lds, seg, st);
 }
 
 // Prefetch depth (nodes in flight per stream) by labels per lane; the row
@@ -798,15 +885,22 @@ static void launch_lpl(ChainArgs& a, size_t lds, hipStream_t st) {
 #endif
 
 template <int MODE>
-static int launch_mode(ChainArgs& a, hipStream_t st) {
+static int launch_mode(ChainArgs& a, bool seg, hipStream_t st) {
   const size_t lds = (sizeof(double) + 2 * sizeof(float)) * (size_t)a.n;
   constexpr int P2 = MODE == kColsMinorant ? CCS_CRF_P_COLS : CCS_CRF_P_ROWS;
-  if (a.D <= 64) launch_lpl<MODE, 1, P2>(a, lds, st);
-  else if (a.D <= 128) launch_lpl<MODE, 2, P2>(a, lds, st);
-  else launch_lpl<MODE, 4, (P2 / 2 > 4 ? P2 / 2 : 4)>(a, lds, st);
+  if (a.D <= 64) launch_lpl<MODE, 1, P2>(a, lds, seg, st);
+  else if (a.D <= 128) launch_lpl<MODE, 2, P2>(a, lds, seg, st);
+  else launch_lpl<MODE, 4, (P2 / 2 > 4 ? P2 / 2 : 4)>(a, lds, seg, st);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }
+
+// Volumes from which the column sweeps recompute messages from checkpoints.
+// Below it the sweep is latency-bound and the recomputation only adds steps;
+// above it the sweep is HBM-bound and the recomputation removes two of the
+// seven volume passes (measured: 0.304 -> 0.228 ms at 1242x375x128, 1.93 ->
+// 1.55 ms at 1488x992x256; 0.056 -> 0.060 ms at 300x200x64).
+constexpr long kSegMinBytes = 64l << 20;
 
 int launch_chain(ChainMode mode, ChainArgs& a, hipStream_t st) {
   if (mode != kColsMinorant) {  // row chains are addressed with 32-bit byte offsets
@@ -815,10 +909,17 @@ int launch_chain(ChainMode mode, ChainArgs& a, hipStream_t st) {
     if (extent > 0x7fffffffL || msg > 0x7fffffffL)
       return fail(CCS_ERR_ARG, "row chain of %d nodes x %d labels exceeds 2 GiB", a.n, a.D);
   }
+  bool seg = false;
+  if (mode == kColsMinorant && kSeg > 0) {
+    // A segment's nodes are addressed with 32-bit byte offsets from its first node.
+    const long span = ((long)(kSeg - 1) * a.pns * a.D + a.D) * (long)sizeof(float);
+    const long vol = (long)a.n_chains * a.n * a.D * (long)sizeof(float);
+    seg = span <= 0x7fffffffL && vol >= kSegMinBytes;
+  }
   switch (mode) {
-    case kRowsMinorant: return launch_mode<kRowsMinorant>(a, st);
-    case kColsMinorant: return launch_mode<kColsMinorant>(a, st);
-    case kRowsDecode: return launch_mode<kRowsDecode>(a, st);
+    case kRowsMinorant: return launch_mode<kRowsMinorant>(a, false, st);
+    case kColsMinorant: return launch_mode<kColsMinorant>(a, seg, st);
+    case kRowsDecode: return launch_mode<kRowsDecode>(a, false, st);
   }
   return fail(CCS_ERR_ARG, "unknown chain mode %d", (int)mode);
 }

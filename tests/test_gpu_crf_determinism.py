@@ -1,0 +1,57 @@
+"""Run-to-run determinism of the SlackProp sweeps at sizes where the column
+sweep recomputes its messages from checkpoints (volumes of 64 MiB and more).
+
+Each case runs rows, columns, rows, columns and the decode on fresh handles
+(so the first call of a handle, which allocates its message scratch, is
+included) and on a handle's second call, and requires the unaries, the float64
+chain/kept sums and the labels to be bit-identical.  At D = 256 every lane
+holds four labels; these sweeps once stored 128-bit vectors whose data
+registers the next instruction overwrote, which showed up only here, as
+results that changed from call to call.  A full-size case at 1488 x 992 x 256
+and the 1242 x 375 x 128 benchmark volume are included.
+"""
+import pytest
+import torch
+
+from cnn_crf_stereo_amd import capi, ops
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(992, 300, 256), (200, 1488, 256), (375, 1242, 128), (992, 1488, 256)]
+
+
+def run_sequence(h, cost, wx, wy):
+    H, W, D = cost.shape
+    A = cost.clone()
+    sums = torch.zeros(8, dtype=torch.float64, device="cuda")
+    k = 0
+    for _ in range(2):
+        for d in (0, 1):
+            ops.slackprop_sweep(h, cost, A, wx, wy, 0.5, 2.0, 2, d, chain_sum=sums[k:k + 1],
+                                kept_sum=sums[k + 4:k + 5])
+            k += 1
+    labels = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    ops.slackprop_sweep(h, cost, A, wx, wy, 0.5, 2.0, 2, 2, labels=labels)
+    torch.cuda.synchronize()
+    return A, sums, labels
+
+
+@pytest.mark.parametrize("H,W,D", SHAPES)
+def test_sweeps_bit_identical_across_handles_and_calls(H, W, D):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(H * 7 + W + D)
+    cost = -torch.softmax(torch.randn(H, W, D, device="cuda", generator=g) * 3, dim=-1)
+    wx = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    wy = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    runs = []
+    for _ in range(2):
+        h = capi.Handle()
+        runs.append(run_sequence(h, cost, wx, wy))  # first call of the handle
+        runs.append(run_sequence(h, cost, wx, wy))
+        h.close()
+    A0, s0, l0 = runs[0]
+    assert bool(torch.isfinite(A0).all())
+    for A, s, lab in runs[1:]:
+        assert torch.equal(A.view(torch.int32), A0.view(torch.int32))
+        assert torch.equal(s, s0)
+        assert torch.equal(lab, l0)
