@@ -9,8 +9,11 @@ CNN of 3x3 layers and 8 CRF iterations; the BF16-MFMA form is the measured
 value and the FP32 form runs beside it on the same GPU.
 
 Prints one JSON line (see --help).  Per-launch HIP events bracket every kernel
-launch inside the timed steps on the handle's stream; the roofline entry is
-the dominant kernel's algorithmic work per launch over its mean duration.
+launch inside the timed steps on the handle's stream; per-launch figures (the
+roofline entry, stages_ms, kernels) come from a pass with both towers on one
+stream, so that no event interval spans the other tower's launches.  The
+roofline entry is the dominant kernel class's algorithmic work per launch over
+its mean duration.
 """
 import argparse
 import importlib.util
@@ -363,6 +366,12 @@ def main():
         side = (capi.Handle(s2), s2)
     ms_bf16, launches_bf16, out_bf16 = time_form(pkg, h, data, spec, params, "bf16", args.steps, args.warmup, side)
     result_extra = {"pcie_inclusive_ms_per_frame": round(launches_bf16.pop("_pcie_inclusive_ms"), 4)}
+    if side is not None:
+        # Per-launch figures come from a one-stream pass: with the towers on two
+        # streams each event interval also spans the other tower's launches.
+        ms_1s, launches_bf16, _ = time_form(pkg, h, data, spec, params, "bf16", args.steps, 1, None)
+        launches_bf16.pop("_pcie_inclusive_ms")
+        result_extra["ms_per_frame_one_stream"] = round(ms_1s, 4)
     if args.inflight > 1:
         lanes = []
         for _ in range(args.inflight):
@@ -380,10 +389,14 @@ def main():
         ms_fp32, launches_fp32, out_fp32 = time_form(pkg, h, data, spec, params, "fp32", args.steps, args.warmup,
                                                      side)
         launches_fp32.pop("_pcie_inclusive_ms")
+        if side is not None:
+            _, launches_fp32, _ = time_form(pkg, h, data, spec, params, "fp32", max(2, args.steps // 2), 1, None)
+            launches_fp32.pop("_pcie_inclusive_ms")
         result_extra["fp32_ms_per_frame"] = round(ms_fp32, 4)
         result_extra["fp32_pixel_disp_per_s"] = pd_per_frame / (ms_fp32 / 1e3)
         result_extra["speedup_bf16_vs_fp32"] = round(ms_fp32 / ms_bf16, 3)
-        result_extra["stages_ms_fp32"] = {k: round(v["ms"] / args.steps, 4) for k, v in launches_fp32.items()}
+        nf = max(2, args.steps // 2) if side is not None else args.steps
+        result_extra["stages_ms_fp32"] = {k: round(v["ms"] / nf, 4) for k, v in launches_fp32.items()}
 
     gt = data["disp"]
     bad = {"bf16": bad_pixel_rate(out_bf16["labels"].cpu().numpy(), gt, D)}
