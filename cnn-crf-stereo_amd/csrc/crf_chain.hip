@@ -94,6 +94,33 @@ __device__ __forceinline__ Vec<LPL> shift_label(const Vec<LPL>& x, int dir, floa
   return o;
 }
 
+// The destinations of the one-label lane shifts, carried from step to step.  A
+// shift by one lane writes every lane but the boundary one (lane 0 for the
+// shift towards higher labels, lane 63 for the other), which keeps the value of
+// the destination register; set to the fill once, that lane holds the fill for
+// good, and no step re-initialises it.  mn_*: fill +inf (minimum of the
+// neighbours), mx_*: fill -inf (maximum).
+struct ShiftCarry {
+  float mn_prev = kInfF, mn_next = kInfF, mx_prev = -kInfF, mx_next = -kInfF;
+};
+
+template <int LPL>
+__device__ __forceinline__ Vec<LPL> shift_carry(const Vec<LPL>& x, int dir, float& carry) {
+  Vec<LPL> o;
+  if (dir < 0) {
+    carry = dpp_f<0x138, 0xf, 0xf, false>(carry, x.v[LPL - 1]);  // wave_shr:1
+    o.v[0] = carry;
+#pragma unroll
+    for (int j = 1; j < LPL; ++j) o.v[j] = x.v[j - 1];
+  } else {
+    carry = dpp_f<0x130, 0xf, 0xf, false>(carry, x.v[0]);  // wave_shl:1
+    o.v[LPL - 1] = carry;
+#pragma unroll
+    for (int j = 0; j + 1 < LPL; ++j) o.v[j] = x.v[j + 1];
+  }
+  return o;
+}
+
 template <int LPL>
 __device__ __forceinline__ Vec<LPL> plus(const Vec<LPL>& x, const Vec<LPL>& y) {
   Vec<LPL> o;
@@ -137,9 +164,9 @@ __device__ __forceinline__ void pin(Vec<LPL>& x) {
 // near(s) = min over 0 < |s - t| < delta of x(t)  (+inf if delta == 1).
 // DC = 2 fixes delta = 2 at compile time (no branch in the step); DC = 0 reads it.
 template <int LPL, int DC>
-__device__ __forceinline__ Vec<LPL> near_min(const Vec<LPL>& x, int delta) {
+__device__ __forceinline__ Vec<LPL> near_min(const Vec<LPL>& x, int delta, ShiftCarry& c) {
   if (DC == 2 || delta == 2) {  // the common case, straight-line
-    const Vec<LPL> l = shift_label(x, -1, kInfF), r = shift_label(x, +1, kInfF);
+    const Vec<LPL> l = shift_carry(x, -1, c.mn_prev), r = shift_carry(x, +1, c.mn_next);
     Vec<LPL> o;
 #pragma unroll
     for (int j = 0; j < LPL; ++j) o.v[j] = fminf(l.v[j], r.v[j]);
@@ -159,9 +186,9 @@ __device__ __forceinline__ Vec<LPL> near_min(const Vec<LPL>& x, int delta) {
 }
 
 template <int LPL, int DC>
-__device__ __forceinline__ Vec<LPL> near_max(const Vec<LPL>& x, int delta) {
+__device__ __forceinline__ Vec<LPL> near_max(const Vec<LPL>& x, int delta, ShiftCarry& c) {
   if (DC == 2 || delta == 2) {
-    const Vec<LPL> l = shift_label(x, -1, -kInfF), r = shift_label(x, +1, -kInfF);
+    const Vec<LPL> l = shift_carry(x, -1, c.mx_prev), r = shift_carry(x, +1, c.mx_next);
     Vec<LPL> o;
 #pragma unroll
     for (int j = 0; j < LPL; ++j) o.v[j] = fmaxf(l.v[j], r.v[j]);
@@ -183,8 +210,8 @@ __device__ __forceinline__ Vec<LPL> near_max(const Vec<LPL>& x, int delta) {
 // m(s) = min(U(s), near(U)(s) + w L1, min U + w L2).
 template <int LPL, int DC>
 __device__ __forceinline__ Vec<LPL> minconv(const Vec<LPL>& U, float umin, float w, float L1, float L2,
-                                            int delta) {
-  const Vec<LPL> nr = near_min<LPL, DC>(U, delta);
+                                            int delta, ShiftCarry& c) {
+  const Vec<LPL> nr = near_min<LPL, DC>(U, delta, c);
   const float far = umin + w * L2;
   const float wl1 = w * L1;
   Vec<LPL> m;
@@ -373,6 +400,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
   const int n = a.n, D = a.D;
+  ShiftCarry cc;
   const long pbase = (long)c * a.pcs;
   const long nstride = a.pns * (long)D;  // floats between consecutive nodes
   const float* __restrict__ Ab = a.A + pbase * D;
@@ -433,7 +461,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   auto step_b = [&](int s, const Vec<LPL>& U, float wi) {
     const int i = n - 2 - s;
     const float u = wave_min(local_min(U));
-    const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, a.L1, a.L2, a.delta);
+    const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, a.L1, a.L2, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) bh.v[k] = valid[k] ? m.v[k] - u : 0.f;
     if constexpr (kCols) {
@@ -509,17 +537,18 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       }
       const float v = wave_min(local_min(V));
       u_lds[i] = v;  // every lane writes the same value
-      const Vec<LPL> m = minconv<LPL, DC>(V, v, wi, a.L1, a.L2, a.delta);
+      const Vec<LPL> m = minconv<LPL, DC>(V, v, wi, a.L1, a.L2, a.delta, cc);
       Vec<LPL> mm;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) mm.v[k] = valid[k] ? m.v[k] : -kInfF;
-      const Vec<LPL> nmax = near_max<LPL, DC>(mm, a.delta);
+      const Vec<LPL> nmax = near_max<LPL, DC>(mm, a.delta, cc);
       const float wl1 = wi * a.L1;
       Vec<LPL> out;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) {
-        const float e = valid[k] ? V.v[k] - fmaxf(m.v[k], nmax.v[k] - wl1) : 0.f;
-        const float g = gt.v[k] + e;
+        // g = gamma (R + Bh) + excess, excess = V - max(m, near_max(m) - w L1); with
+        // V = R - gamma (R + Bh) that is R - max(...), formed directly.
+        const float g = valid[k] ? R.v[k] - fmaxf(m.v[k], nmax.v[k] - wl1) : 0.f;
         out.v[k] = MODE == kRowsMinorant ? a_cur.v[k] - g : a_cur.v[k] + g;
         R.v[k] = valid[k] ? thn.v[k] + m.v[k] - v : kInfF;
       }
@@ -573,7 +602,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
           if (t < cnt) {
             const Vec<LPL> U = plus(theta_of(B.a[t], B.c[t]), Xb);
             const float u = lane_weight(B.u, t);
-            const Vec<LPL> m = minconv<LPL, DC>(U, u, lane_weight(B.w, t), a.L1, a.L2, a.delta);
+            const Vec<LPL> m = minconv<LPL, DC>(U, u, lane_weight(B.w, t), a.L1, a.L2, a.delta, cc);
 #pragma unroll
             for (int k = 0; k < LPL; ++k) Xb.v[k] = valid[k] ? m.v[k] - u : 0.f;
             bs[t] = Xb;
@@ -712,6 +741,7 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform branch
   const int lane = threadIdx.x & 63;
   const int n = a.n, D = a.D;
+  ShiftCarry cc;
   const long pbase = (long)c * a.pcs;
   const long nstride = a.pns * (long)D;
   const float* __restrict__ Ab = a.A + pbase * D;
@@ -732,7 +762,7 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   Vec<LPL> X;  // wave 0: F_i, wave 1: Bh_i
   auto fwd = [&](const Vec<LPL>& an, float w) {  // F_i = A_i + minconv(F_{i-1}; w_{i-1}) - min F_{i-1}
     const float f = wave_min(local_min(X));
-    const Vec<LPL> m = minconv<LPL, DC>(X, f, w, a.L1, a.L2, a.delta);
+    const Vec<LPL> m = minconv<LPL, DC>(X, f, w, a.L1, a.L2, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
   };
@@ -741,7 +771,7 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
 #pragma unroll
     for (int k = 0; k < LPL; ++k) U.v[k] = (valid[k] ? an1.v[k] : kInfF) + X.v[k];
     const float u = wave_min(local_min(U));
-    const Vec<LPL> m = minconv<LPL, DC>(U, u, w, a.L1, a.L2, a.delta);
+    const Vec<LPL> m = minconv<LPL, DC>(U, u, w, a.L1, a.L2, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? m.v[k] - u : 0.f;
     u_lds[i] = u;  // every lane writes the same value
