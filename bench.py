@@ -123,13 +123,12 @@ def run_frame(ops, h, frame, left, right, L, A, side=None):
     wx, wy = L.run("edge_weights", {"bytes": 12.0 * H * W},
                    lambda: ops.edge_weights(h, left[0], frame.sigma, frame.alpha, frame.beta, frame.lam))
     V = 4.0 * H * W * D
-    A.copy_(cost)
-    import torch
-
     labels = torch.empty((H, W), dtype=torch.int32, device="cuda")
-    for _ in range(frame.iters):
-        L.run("crf_rows", {"bytes": 2 * V}, lambda: ops.slackprop_sweep(h, cost, A, wx, wy, frame.L1, frame.L2,
-                                                                        frame.delta, 0))
+    for it in range(frame.iters):
+        # The first row sweep reads its unaries from the cost volume (direction 3).
+        L.run("crf_rows", {"bytes": 2 * V},
+              lambda it=it: ops.slackprop_sweep(h, cost, A, wx, wy, frame.L1, frame.L2, frame.delta,
+                                                3 if it == 0 else 0))
         L.run("crf_cols", {"bytes": 3 * V}, lambda: ops.slackprop_sweep(h, cost, A, wx, wy, frame.L1, frame.L2,
                                                                         frame.delta, 1))
     L.run("crf_decode", {"bytes": V + 4.0 * H * W},

@@ -175,8 +175,10 @@ extern "C" int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, 
   CCS_CHECK(h && A, CCS_ERR_ARG, "ccs_slackprop_sweep: NULL argument");
   int rc = check_crf_args(H, W, D, L1, L2, delta);
   if (rc) return rc;
-  CCS_CHECK(direction >= 0 && direction <= 2, CCS_ERR_ARG, "ccs_slackprop_sweep: direction must be 0, 1 or 2");
-  CCS_CHECK(direction != 0 || wx, CCS_ERR_ARG, "ccs_slackprop_sweep: rows need wx");
+  CCS_CHECK(direction >= 0 && direction <= 3, CCS_ERR_ARG, "ccs_slackprop_sweep: direction must be 0, 1, 2 or 3");
+  CCS_CHECK((direction != 0 && direction != 3) || wx, CCS_ERR_ARG, "ccs_slackprop_sweep: rows need wx");
+  CCS_CHECK(direction != 3 || (cost && cost != A), CCS_ERR_ARG,
+            "ccs_slackprop_sweep: the first row sweep reads cost, which must not be A");
   CCS_CHECK(direction != 1 || (wy && cost), CCS_ERR_ARG, "ccs_slackprop_sweep: columns need cost and wy");
   CCS_CHECK(direction != 2 || (wx && labels), CCS_ERR_ARG, "ccs_slackprop_sweep: decode needs wx and labels");
   const size_t vol = (size_t)H * W * D;
@@ -186,14 +188,15 @@ extern "C" int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, 
   double* lam = (double*)h->crf_small.ptr;
   double* kept = lam + nmax;
   ChainArgs a{};
-  a.cost = cost; a.A = A; a.msg = (float*)h->crf_msg.ptr; a.lam = lam; a.kept = kept;
+  a.cost = cost; a.A = A; a.A_in = direction == 3 ? cost : A;
+  a.msg = (float*)h->crf_msg.ptr; a.lam = lam; a.kept = kept;
   a.D = D; a.delta = delta; a.L1 = L1; a.L2 = L2;
   if (direction == 1) {
     a.w = wy; a.pcs = 1; a.pns = W; a.n_chains = W; a.n = H;
   } else {
     a.w = wx; a.pcs = W; a.pns = 1; a.n_chains = H; a.n = W;
   }
-  if (direction == 0) rc = launch_chain(kRowsMinorant, a, h->stream);
+  if (direction == 0 || direction == 3) rc = launch_chain(kRowsMinorant, a, h->stream);
   else if (direction == 1) rc = launch_chain(kColsMinorant, a, h->stream);
   else {
     a.labels = labels;
@@ -219,9 +222,11 @@ extern "C" int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx,
   float* A = (float*)h->crf_a.ptr;
   double* S = (double*)h->crf_sums.ptr;  // S_0 .. S_{2 iters}
   double* K = S + 2 * iters + 1;         // K_0 .. K_{2 iters - 1}
-  CCS_HIP(hipMemcpyAsync(A, cost, vol * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+  if (iters == 0) CCS_HIP(hipMemcpyAsync(A, cost, vol * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
   for (int it = 0; it < iters; ++it) {
-    if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, 0, nullptr, S + 2 * it, K + 2 * it)))
+    // The first row sweep reads the unaries from cost (A = cost is never copied).
+    if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, it == 0 ? 3 : 0, nullptr, S + 2 * it,
+                                  K + 2 * it)))
       return rc;
     if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, 1, nullptr, S + 2 * it + 1,
                                   K + 2 * it + 1)))
@@ -248,7 +253,7 @@ extern "C" int ccs_slackprop_decode_gap(ccs_handle_t h, const float* unaries, co
   if ((rc = h->crf_msg.reserve(vol * sizeof(float)))) return rc;
   if ((rc = h->crf_small.reserve(sizeof(double) * 2 * (size_t)H))) return rc;
   ChainArgs dec{};
-  dec.A = const_cast<float*>(unaries); dec.w = wx; dec.msg = (float*)h->crf_msg.ptr;
+  dec.A = const_cast<float*>(unaries); dec.A_in = unaries; dec.w = wx; dec.msg = (float*)h->crf_msg.ptr;
   dec.labels = labels; dec.gap = gap; dec.lam = (double*)h->crf_small.ptr; dec.kept = dec.lam + H;
   dec.pcs = W; dec.pns = 1; dec.n_chains = H; dec.n = W; dec.D = D; dec.delta = delta; dec.L1 = L1; dec.L2 = L2;
   return launch_chain(kRowsDecode, dec, h->stream);

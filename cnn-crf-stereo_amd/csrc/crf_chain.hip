@@ -403,7 +403,9 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   ShiftCarry cc;
   const long pbase = (long)c * a.pcs;
   const long nstride = a.pns * (long)D;  // floats between consecutive nodes
-  const float* __restrict__ Ab = a.A + pbase * D;
+  // Rows read the unaries from A_in (A, or the cost volume in a run's first
+  // sweep) and write A; columns read and write A.
+  const float* __restrict__ Ab = (kCols ? a.A : a.A_in) + pbase * D;
   float* __restrict__ Aw = a.A + pbase * D;
   const float* __restrict__ Cb = kCols ? a.cost + pbase * D : nullptr;
   const float* __restrict__ wb = a.w + pbase;
@@ -433,6 +435,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   // addresses (their nodes span the whole volume).
   const int strideB = (int)(nstride * 4);
   const Rsrc rA = chain_rsrc(Ab, (long)(n - 1) * nstride + D);
+  const Rsrc rW = chain_rsrc(Aw, (long)(n - 1) * nstride + D);
   const Rsrc rM = chain_rsrc(msg, (long)n * D);
 
   // ---- backward pass: step s handles i = n-2-s: Bh[i] = minconv(theta[i+1] + Bh[i+1]) - min(...)
@@ -557,7 +560,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     };
     auto store_f = [&](int i, const Vec<LPL>& out) {
       if constexpr (kCols) st_node<LPL, FULL>(Aw + i * nstride, lane, D, out);
-      else st_buf<LPL, FULL>(rA, i * strideB, lane, D, out);
+      else st_buf<LPL, FULL>(rW, i * strideB, lane, D, out);
     };
     if constexpr (kCols && SEG && seg_len<LPL>() > 0) {
       // Segments of K forward steps i0 .. i0+K-1.  Their nodes i0+1 .. i0+K

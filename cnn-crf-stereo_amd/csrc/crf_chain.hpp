@@ -13,6 +13,7 @@ enum ChainMode { kRowsMinorant = 0, kColsMinorant = 1, kRowsDecode = 2 };
 struct ChainArgs {
   const float* cost;   // C (column mode)
   float* A;            // row unaries, updated in place
+  const float* A_in;   // where a row sweep reads the unaries (A, or cost for the first sweep)
   const float* w;      // edge weight of the edge (node i, node i+1), indexed by pixel
   float* msg;          // backward messages, [chain][node][D]
   int* labels;         // decode output, indexed by pixel

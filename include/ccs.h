@@ -177,7 +177,8 @@ int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx, const floa
 /* One half-step of ccs_slackprop on caller-held row unaries A (H x W x D, set to
  * a copy of cost before the first call): direction 0 sweeps the rows
  * (A -= minorant), 1 the columns (A += minorant, reads cost), 2 decodes the rows
- * into labels.  chain_sum / kept_sum (optional device doubles) receive S_j / K_j.
+ * into labels; 3 is the first row sweep of a run without that copy (it reads the
+ * unaries from cost and writes A = cost - minorant, so A needs no initialisation).  chain_sum / kept_sum (optional device doubles) receive S_j / K_j.
  * The steps of slack_prop_2D_alg::execute (stereo/stereo.cpp:153), exposed for
  * callers that schedule or time the sweeps themselves. */
 int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, const float* wx, const float* wy,

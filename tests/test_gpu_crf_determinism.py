@@ -55,3 +55,24 @@ def test_sweeps_bit_identical_across_handles_and_calls(H, W, D):
         assert torch.equal(A.view(torch.int32), A0.view(torch.int32))
         assert torch.equal(s, s0)
         assert torch.equal(lab, l0)
+
+
+@pytest.mark.parametrize("H,W,D", [(375, 1242, 128), (61, 97, 100), (40, 300, 256)])
+def test_first_row_sweep_reads_cost(handle, H, W, D):
+    """Direction 3 (unaries read from cost, written to A) equals a copy A = cost
+    followed by direction 0, bit for bit, with A holding garbage beforehand."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(H + W + D)
+    cost = -torch.softmax(torch.randn(H, W, D, device="cuda", generator=g) * 3, dim=-1)
+    wx = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    wy = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    sums = torch.zeros(4, dtype=torch.float64, device="cuda")
+    A0 = cost.clone()
+    ops.slackprop_sweep(handle, cost, A0, wx, wy, 0.5, 2.0, 2, 0, chain_sum=sums[0:1], kept_sum=sums[1:2])
+    A3 = torch.full_like(cost, float("nan"))
+    ops.slackprop_sweep(handle, cost, A3, wx, wy, 0.5, 2.0, 2, 3, chain_sum=sums[2:3], kept_sum=sums[3:4])
+    torch.cuda.synchronize()
+    assert torch.equal(A3.view(torch.int32), A0.view(torch.int32))
+    assert torch.equal(sums[2:4], sums[0:2])
+    with pytest.raises(capi.CCSError):
+        ops.slackprop_sweep(handle, cost, cost, wx, wy, 0.5, 2.0, 2, 3)
