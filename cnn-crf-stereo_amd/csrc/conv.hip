@@ -442,6 +442,11 @@ __global__ __launch_bounds__(256) void k_conv_image(const float* __restrict__ x,
                                                     const float* __restrict__ w, const float* __restrict__ bias,
                                                     int K, int act, TO* __restrict__ y, int Ho, int Wo) {
   constexpr int KMAX = 128;
+  // FP32 outputs (32 bytes per 8 channels) go straight to memory; bf16 outputs
+  // (16 bytes) are staged through LDS and copied out as one contiguous block,
+  // which measured faster for them (0.051 vs 0.053 ms at 1256 x 389; FP32:
+  // 0.080 ms direct, 0.109 ms staged).
+  constexpr bool kDirect = sizeof(TO) == 4;
   __shared__ float ws[C * 9][KMAX];
   __shared__ float bs[KMAX];
   extern __shared__ __align__(16) unsigned char lds_out[];
@@ -481,20 +486,30 @@ __global__ __launch_bounds__(256) void k_conv_image(const float* __restrict__ x,
       acc[0] += in[t] * w0.x; acc[1] += in[t] * w0.y; acc[2] += in[t] * w0.z; acc[3] += in[t] * w0.w;
       acc[4] += in[t] * w1.x; acc[5] += in[t] * w1.y; acc[6] += in[t] * w1.z; acc[7] += in[t] * w1.w;
     }
+    // Eight channels, rounded to TO, as one vector (16 bytes in bf16).
+    typedef __attribute__((ext_vector_type(8))) TO vec8;
+    vec8 o;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       float v = acc[e];
       if (act) v = sizeof(TO) == 2 ? tanh_for_bf16(v) : tanh_fast(v);
-      os[threadIdx.x * Cs + k0 + e] = (TO)(k0 + e < K ? v : 0.f);
+      o[e] = (TO)(k0 + e < K ? v : 0.f);
+    }
+    if constexpr (kDirect) {
+      if (p < P) *reinterpret_cast<vec8*>(y + p * Cs + k0) = o;
+    } else {
+      *reinterpret_cast<vec8*>(os + threadIdx.x * Cs + k0) = o;
     }
   }
-  __syncthreads();
-  // The block's pixels are consecutive in memory: copy 16-byte vectors.
-  const long npx = (P - p0) < 256 ? (P - p0) : 256;
-  const int nvec = (int)(npx * Cs * sizeof(TO) / 16);
-  const uint4* src = reinterpret_cast<const uint4*>(os);
-  uint4* dst = reinterpret_cast<uint4*>(y + p0 * Cs);
-  for (int v = threadIdx.x; v < nvec; v += 256) dst[v] = src[v];
+  if constexpr (!kDirect) {
+    __syncthreads();
+    // The block's pixels are consecutive in memory: copy 16-byte vectors.
+    const long npx = (P - p0) < 256 ? (P - p0) : 256;
+    const int nvec = (int)(npx * Cs * sizeof(TO) / 16);
+    const uint4* src = reinterpret_cast<const uint4*>(os);
+    uint4* dst = reinterpret_cast<uint4*>(y + p0 * Cs);
+    for (int v = threadIdx.x; v < nvec; v += 256) dst[v] = src[v];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -623,10 +638,10 @@ static void launch_image(const float* x, int H, int W, const float* w, const flo
   const int Ho = H - 2, Wo = W - 2;
   const long P = (long)Ho * Wo;
   const int Cs = feature_stride(K);
-  if (precision == CCS_BF16)
+  if (precision == CCS_BF16)  // bf16 outputs are staged in LDS (256 pixels x Cs)
     k_conv_image<C, bf16><<<ceil_div(P, 256), 256, 256 * Cs * 2, st>>>(x, H, W, w, b, K, act, (bf16*)y, Ho, Wo);
   else
-    k_conv_image<C, float><<<ceil_div(P, 256), 256, 256 * Cs * 4, st>>>(x, H, W, w, b, K, act, (float*)y, Ho, Wo);
+    k_conv_image<C, float><<<ceil_div(P, 256), 256, 0, st>>>(x, H, W, w, b, K, act, (float*)y, Ho, Wo);
 }
 
 static int conv_generic(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C, const float* w,

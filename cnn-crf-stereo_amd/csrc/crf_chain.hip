@@ -161,6 +161,15 @@ __device__ __forceinline__ void pin(Vec<LPL>& x) {
   for (int j = 0; j < LPL; ++j) asm volatile("" : "+v"(x.v[j]));
 }
 
+// A kernel-uniform value kept in a VGPR.  The steps multiply the edge weight
+// (an SGPR, taken with v_readlane) by L1 and L2; with L1 and L2 in VGPRs each of
+// those FMAs has one scalar operand, as a VALU instruction allows, and the weight
+// needs no copy into a VGPR per step.
+__device__ __forceinline__ float in_vgpr(float x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // near(s) = min over 0 < |s - t| < delta of x(t)  (+inf if delta == 1).
 // DC = 2 fixes delta = 2 at compile time (no branch in the step); DC = 0 reads it.
 template <int LPL, int DC>
@@ -401,6 +410,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   const int lane = threadIdx.x;
   const int n = a.n, D = a.D;
   ShiftCarry cc;
+  const float L1v = in_vgpr(a.L1), L2v = in_vgpr(a.L2);
   const long pbase = (long)c * a.pcs;
   const long nstride = a.pns * (long)D;  // floats between consecutive nodes
   // Rows read the unaries from A_in (A, or the cost volume in a run's first
@@ -464,7 +474,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   auto step_b = [&](int s, const Vec<LPL>& U, float wi) {
     const int i = n - 2 - s;
     const float u = wave_min(local_min(U));
-    const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, a.L1, a.L2, a.delta, cc);
+    const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, L1v, L2v, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) bh.v[k] = valid[k] ? m.v[k] - u : 0.f;
     if constexpr (kCols) {
@@ -540,12 +550,12 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       }
       const float v = wave_min(local_min(V));
       u_lds[i] = v;  // every lane writes the same value
-      const Vec<LPL> m = minconv<LPL, DC>(V, v, wi, a.L1, a.L2, a.delta, cc);
+      const Vec<LPL> m = minconv<LPL, DC>(V, v, wi, L1v, L2v, a.delta, cc);
       Vec<LPL> mm;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) mm.v[k] = valid[k] ? m.v[k] : -kInfF;
       const Vec<LPL> nmax = near_max<LPL, DC>(mm, a.delta, cc);
-      const float wl1 = wi * a.L1;
+      const float wl1 = wi * L1v;
       Vec<LPL> out;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) {
@@ -605,7 +615,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
           if (t < cnt) {
             const Vec<LPL> U = plus(theta_of(B.a[t], B.c[t]), Xb);
             const float u = lane_weight(B.u, t);
-            const Vec<LPL> m = minconv<LPL, DC>(U, u, lane_weight(B.w, t), a.L1, a.L2, a.delta, cc);
+            const Vec<LPL> m = minconv<LPL, DC>(U, u, lane_weight(B.w, t), L1v, L2v, a.delta, cc);
 #pragma unroll
             for (int k = 0; k < LPL; ++k) Xb.v[k] = valid[k] ? m.v[k] - u : 0.f;
             bs[t] = Xb;
@@ -745,6 +755,7 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   const int lane = threadIdx.x & 63;
   const int n = a.n, D = a.D;
   ShiftCarry cc;
+  const float L1v = in_vgpr(a.L1), L2v = in_vgpr(a.L2);
   const long pbase = (long)c * a.pcs;
   const long nstride = a.pns * (long)D;
   const float* __restrict__ Ab = a.A + pbase * D;
@@ -765,7 +776,7 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   Vec<LPL> X;  // wave 0: F_i, wave 1: Bh_i
   auto fwd = [&](const Vec<LPL>& an, float w) {  // F_i = A_i + minconv(F_{i-1}; w_{i-1}) - min F_{i-1}
     const float f = wave_min(local_min(X));
-    const Vec<LPL> m = minconv<LPL, DC>(X, f, w, a.L1, a.L2, a.delta, cc);
+    const Vec<LPL> m = minconv<LPL, DC>(X, f, w, L1v, L2v, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
   };
@@ -774,7 +785,7 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
 #pragma unroll
     for (int k = 0; k < LPL; ++k) U.v[k] = (valid[k] ? an1.v[k] : kInfF) + X.v[k];
     const float u = wave_min(local_min(U));
-    const Vec<LPL> m = minconv<LPL, DC>(U, u, w, a.L1, a.L2, a.delta, cc);
+    const Vec<LPL> m = minconv<LPL, DC>(U, u, w, L1v, L2v, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? m.v[k] - u : 0.f;
     u_lds[i] = u;  // every lane writes the same value
