@@ -32,6 +32,8 @@
 // moves.  Costs must be finite or +inf; a NaN cost gives unspecified labels
 // (never an out-of-range memory access: labels are only stored, and every
 // address is computed from chain/node/lane indices).
+#include <cstdlib>
+
 #include "crf_chain.hpp"
 
 namespace ccs {
@@ -946,6 +948,17 @@ static int launch_mode(ChainArgs& a, bool seg, hipStream_t st) {
 // 1.55 ms at 1488x992x256; 0.056 -> 0.060 ms at 300x200x64).
 constexpr long kSegMinBytes = 64l << 20;
 
+// The threshold, overridable by the environment variable CCS_CRF_SEG_MIN_BYTES
+// (read once per process; tests use it to run the checkpointed scheme on small
+// volumes and compare it bit for bit with the store-everything one).
+static long seg_min_bytes() {
+  static const long v = [] {
+    const char* e = getenv("CCS_CRF_SEG_MIN_BYTES");
+    return e ? atol(e) : kSegMinBytes;
+  }();
+  return v;
+}
+
 int launch_chain(ChainMode mode, ChainArgs& a, hipStream_t st) {
   if (mode != kColsMinorant) {  // row chains are addressed with 32-bit byte offsets
     const long extent = ((long)(a.n - 1) * a.pns * a.D + a.D) * (long)sizeof(float);
@@ -958,7 +971,7 @@ int launch_chain(ChainMode mode, ChainArgs& a, hipStream_t st) {
     // A segment's nodes are addressed with 32-bit byte offsets from its first node.
     const long span = ((long)(kSeg - 1) * a.pns * a.D + a.D) * (long)sizeof(float);
     const long vol = (long)a.n_chains * a.n * a.D * (long)sizeof(float);
-    seg = span <= 0x7fffffffL && vol >= kSegMinBytes;
+    seg = span <= 0x7fffffffL && vol >= seg_min_bytes();
   }
   switch (mode) {
     case kRowsMinorant: return launch_mode<kRowsMinorant>(a, false, st);

@@ -76,3 +76,75 @@ def test_first_row_sweep_reads_cost(handle, H, W, D):
     assert torch.equal(sums[2:4], sums[0:2])
     with pytest.raises(capi.CCSError):
         ops.slackprop_sweep(handle, cost, cost, wx, wy, 0.5, 2.0, 2, 3)
+
+
+# Shapes below the checkpointing threshold, covering the kernel variants that the
+# large benchmark volumes do not: labels that do not fill a lane's slots
+# (D = 37, 100, 200), delta != 2, and 1, 2 and 4 labels per lane.
+SMALL = [(97, 131, 100, 2), (120, 90, 37, 3), (64, 70, 256, 4), (150, 257, 200, 2), (61, 97, 64, 2)]
+
+_CHILD = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, sys.argv[1])
+import conftest  # loads the package
+from cnn_crf_stereo_amd import capi
+from test_gpu_crf_determinism import SMALL, small_case, run_small
+h = capi.Handle()
+out = {}
+for (H, W, D, delta) in SMALL:
+    A, sums, labels = run_small(h, *small_case(H, W, D), delta)
+    out["%d_%d_%d_%d_A" % (H, W, D, delta)] = A.cpu().numpy()
+    out["%d_%d_%d_%d_s" % (H, W, D, delta)] = sums.cpu().numpy()
+    out["%d_%d_%d_%d_l" % (H, W, D, delta)] = labels.cpu().numpy()
+np.savez(sys.argv[2], **out)
+print("child ok")
+"""
+
+
+def small_case(H, W, D):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(H * 3 + W + D)
+    cost = -torch.softmax(torch.randn(H, W, D, device="cuda", generator=g) * 3, dim=-1)
+    wx = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    wy = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    return cost, wx, wy
+
+
+def run_small(h, cost, wx, wy, delta):
+    H, W, D = cost.shape
+    A = torch.empty_like(cost)
+    sums = torch.zeros(8, dtype=torch.float64, device="cuda")
+    for it in range(2):
+        ops.slackprop_sweep(h, cost, A, wx, wy, 0.5, 2.0, delta, 3 if it == 0 else 0,
+                            chain_sum=sums[2 * it:2 * it + 1], kept_sum=sums[4 + 2 * it:5 + 2 * it])
+        ops.slackprop_sweep(h, cost, A, wx, wy, 0.5, 2.0, delta, 1,
+                            chain_sum=sums[2 * it + 1:2 * it + 2], kept_sum=sums[5 + 2 * it:6 + 2 * it])
+    labels = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    ops.slackprop_sweep(h, cost, A, wx, wy, 0.5, 2.0, delta, 2, labels=labels)
+    torch.cuda.synchronize()
+    return A, sums, labels
+
+
+def test_checkpointed_columns_equal_stored_messages(handle, tmp_path):
+    """The column sweeps' checkpointed scheme (forced on by the threshold
+    override in a child process) gives the store-everything scheme's unaries,
+    sums and labels bit for bit, on shapes only the latter runs by default."""
+    import os
+    import subprocess
+    import sys
+
+    import numpy as np
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = str(tmp_path / "seg.npz")
+    env = dict(os.environ, CCS_CRF_SEG_MIN_BYTES="0")
+    r = subprocess.run([sys.executable, "-c", _CHILD, here, path], env=env, cwd=here, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    got = np.load(path)
+    for (H, W, D, delta) in SMALL:
+        A, sums, labels = run_small(handle, *small_case(H, W, D), delta)
+        key = "%d_%d_%d_%d_" % (H, W, D, delta)
+        assert np.array_equal(got[key + "A"].view(np.int32), A.cpu().numpy().view(np.int32)), key
+        assert np.array_equal(got[key + "s"], sums.cpu().numpy()), key
+        assert np.array_equal(got[key + "l"], labels.cpu().numpy()), key
