@@ -137,22 +137,22 @@ class SoftmaxXent(torch.autograd.Function):
 
 
 class SsvmHinge(torch.autograd.Function):
-    """Structured-SVM hinge  E(x*) - min_x [E(x) - Delta(x, x*)]  with the
-    minimum taken by SlackProp on loss-augmented unaries; params = (L1, L2).
-    target must hold a label in [0, D) at every pixel."""
+    """Structured-SVM hinge  E(x*) - [E_aug(xhat)]  with xhat the SlackProp
+    minimum of the loss-augmented energy; params = (L1, L2).  Pixels without
+    ground truth (target outside [0, D)) drop out of both energies with their
+    unaries and their edges (the masked energies of ccs_crf_ssvm_grad)."""
 
     @staticmethod
     def forward(ctx, cost, wx, wy, params, target, iters, delta, margin, h):
         H, W, D = cost.shape
-        if bool(((target < 0) | (target >= D)).any()):
-            raise ValueError("SsvmHinge: every pixel needs a ground-truth label in [0, D)")
         L1, L2 = float(params[0]), float(params[1])
         aug = torch.empty_like(cost)
         call("ccs_crf_loss_augment", h.h, ptr(cost), ptr(target), H * W, D, float(margin), ptr(aug))
         r = ops.slackprop(h, aug, wx, wy, L1, L2, delta, iters)
         xhat = r["labels"]
-        e_star = ops.crf_energy(h, cost, wx, wy, target, L1, L2, delta)
-        e_hat = ops.crf_energy(h, aug, wx, wy, xhat, L1, L2, delta)
+        xstar = target.clamp(0, D - 1)  # (only read where target is valid: the mask drops the rest)
+        e_star = ops.crf_energy(h, cost, wx, wy, xstar, L1, L2, delta, mask=target)
+        e_hat = ops.crf_energy(h, aug, wx, wy, xhat, L1, L2, delta, mask=target)
         ctx.save_for_backward(xhat, target, wx, wy)
         ctx.args = (D, L1, L2, delta, h)
         return (e_star - e_hat).float().reshape(())
@@ -202,7 +202,6 @@ def frame_loss(model, h, left, right, target, D, iters=4, delta=2, hinge_weight=
     loss = SoftmaxXent.apply(S, target, h) / n
     if hinge_weight:
         cost = NegSoftmax.apply(S, h).detach()
-        target = target.clamp(0, D - 1)  # pixels without ground truth take label 0 in the hinge
         wx, wy = ops.edge_weights(h, left[0], sigma, alpha, beta, lam)
         loss = loss + hinge_weight * SsvmHinge.apply(cost, wx, wy, model.crf, target, iters, delta, 1.0, h) / (H * W)
     return loss

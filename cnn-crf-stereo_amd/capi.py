@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libccs.so")
 
 FP32, BF16 = 0, 1
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC, ERR_IO, ERR_SHAPE, ERR_NAN = 0, -1, -2, -3, -4, -5, -6, -7
 LAYOUT_NCHW, LAYOUT_NHWC = 0, 1
 EPILOGUE_RAW, EPILOGUE_NEG_SOFTMAX = 0, 1
 
@@ -26,6 +27,7 @@ SIGNATURES = {
     "ccs_destroy": (_I, [_P]),
     "ccs_set_stream": (_I, [_P, _P]),
     "ccs_synchronize": (_I, [_P]),
+    "ccs_reserve": (_I, [_P, _I, _I, _I, _I, _I]),
     "ccs_feature_stride": (_I, [_I]),
     "ccs_pad_normalize": (_I, [_P, _P, _I, _I, _I, _I, _P, _P]),
     "ccs_rgb_to_gray": (_I, [_P, _P, _I, _I, _P]),
@@ -41,10 +43,14 @@ SIGNATURES = {
     "ccs_softmax_xent": (_I, [_P, _P, _I, _I, _P, _P, _P]),
     "ccs_edge_weights": (_I, [_P, _P, _I, _I, _F, _F, _F, _F, _P, _P]),
     "ccs_slackprop_forward": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _F, _I, ctypes.POINTER(_F)]),
+    "ccs_slackprop_forward_volume_out": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _F, _I, _I,
+                                              ctypes.POINTER(_F)]),
     "ccs_slackprop": (_I, [_P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _I, _P, _P, _P, _P]),
+    "ccs_slackprop_status": (_I, [_P]),
     "ccs_slackprop_decode_gap": (_I, [_P, _P, _P, _I, _I, _I, _F, _F, _I, _P, _P]),
     "ccs_slackprop_sweep": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _I, _P, _P, _P]),
     "ccs_crf_energy": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _F, _F, _I, _P]),
+    "ccs_crf_energy_masked": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _F, _F, _I, _P]),
     "ccs_crf_loss_augment": (_I, [_P, _P, _P, ctypes.c_long, _I, _F, _P]),
     "ccs_crf_ssvm_grad": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _P, _P, _P, _P]),
     "ccs_bias_forward": (_I, [_P, _P, _I, _I, _I, _I, _I, _P]),
@@ -67,7 +73,9 @@ _lib = None
 
 
 class CCSError(RuntimeError):
-    pass
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
 
 
 def lib():
@@ -93,7 +101,7 @@ def last_error():
 def call(name, *args):
     rc = getattr(lib(), name)(*args)
     if rc != 0:
-        raise CCSError("%s failed (status %d): %s" % (name, rc, last_error()))
+        raise CCSError("%s failed (status %d): %s" % (name, rc, last_error()), rc)
     return rc
 
 

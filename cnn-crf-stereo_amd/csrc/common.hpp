@@ -38,12 +38,13 @@ int fail(int status, const char* fmt, ...);
                          hipGetErrorString(e_), __FILE__, __LINE__);                   \
   } while (0)
 
-// Grow-only device buffer owned by a handle.
+// Grow-only device buffer owned by a handle, allocated and released in the
+// order of the handle's stream.
 struct Scratch {
   void* ptr = nullptr;
   size_t bytes = 0;
-  int reserve(size_t need);
-  void release();
+  int reserve(size_t need, hipStream_t st);
+  void release(hipStream_t st);
 };
 
 }  // namespace ccs
@@ -59,11 +60,19 @@ struct ccs_handle_s {
   ccs::Scratch crf_sums;   // per-half-step sums of ccs_slackprop
   ccs::Scratch misc;       // small temporaries
   ccs::Scratch misc2;
+  ccs::Scratch crf_nan;    // per row chain: NaN seen by the first row sweep of the last run
+  int nan_chains = 0;      // rows flagged in crf_nan by that sweep (0: none ran)
 };
 
 namespace ccs {
 
 inline int feature_stride(int C) { return (C + 7) / 8 * 8; }
+
+// Blocks of the image-moment pass (one float64 partial pair per block and
+// channel) and of the float64 energy / loss reductions; ccs_reserve sizes the
+// handle's scratch from these.
+constexpr int kMomentBlocks = 128;
+constexpr int kReduceBlocks = 256;
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 

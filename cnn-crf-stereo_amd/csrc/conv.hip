@@ -710,7 +710,7 @@ extern "C" int ccs_conv2d_forward(ccs_handle_t h, const void* x, int x_layout, i
   if (rc) return rc;
   const ConvShape s{H, W, C, K, KH, KW};
   if (x_layout == CCS_LAYOUT_NHWC && mfma_supported(s)) {
-    if ((rc = h->conv_w.reserve(packed_bytes(K, C, KH, KW, precision)))) return rc;
+    if ((rc = h->conv_w.reserve(packed_bytes(K, C, KH, KW, precision), h->stream))) return rc;
     if ((rc = ccs_conv_pack_weights(h, w, K, C, KH, KW, precision, h->conv_w.ptr))) return rc;
     launch_mfma(s, precision, x, h->conv_w.ptr, b, act, y, h->stream);
     CCS_LAUNCH_CHECK();

@@ -412,7 +412,7 @@ extern "C" int ccs_correlation_forward(ccs_handle_t h, const float* input0, cons
   std::vector<int> disp;
   int rc = fetch_disparities(h, disparities, len_disps, disp);
   if (rc) return rc;
-  if ((rc = h->misc.reserve(sizeof(int) * len_disps))) return rc;
+  if ((rc = h->misc.reserve(sizeof(int) * len_disps, h->stream))) return rc;
   int* dd = (int*)h->misc.ptr;
   CCS_HIP(hipMemcpyAsync(dd, disp.data(), sizeof(int) * len_disps, hipMemcpyHostToDevice, h->stream));
   const long total = (long)in * ih * iw * len_disps;
@@ -435,7 +435,7 @@ extern "C" int ccs_correlation_backward(ccs_handle_t h, float* out_grad0, float*
   std::vector<int> disp;
   int rc = fetch_disparities(h, disparities, num_disps, disp);
   if (rc) return rc;
-  if ((rc = h->misc.reserve(sizeof(int) * num_disps))) return rc;
+  if ((rc = h->misc.reserve(sizeof(int) * num_disps, h->stream))) return rc;
   int* dd = (int*)h->misc.ptr;
   CCS_HIP(hipMemcpyAsync(dd, disp.data(), sizeof(int) * num_disps, hipMemcpyHostToDevice, h->stream));
   const long total = (long)n * c * ih * iw;
@@ -463,9 +463,9 @@ extern "C" int ccs_correlation_volume_backward(ccs_handle_t h, const float* f0, 
 extern "C" int ccs_softmax_xent(ccs_handle_t h, const float* S, int HW, int D, const int* target, double* loss,
                                 float* dS) {
   CCS_CHECK(h && S && target && loss && dS && HW >= 1 && D >= 1, CCS_ERR_ARG, "ccs_softmax_xent: bad argument");
-  const int nb = 256;
+  const int nb = kReduceBlocks;
   int rc;
-  if ((rc = h->misc2.reserve(sizeof(double) * nb))) return rc;
+  if ((rc = h->misc2.reserve(sizeof(double) * nb, h->stream))) return rc;
   double* part = (double*)h->misc2.ptr;
   k_softmax_xent<<<nb, 256, 0, h->stream>>>(S, HW, D, target, dS, part);
   k_sum_f64_corr<<<1, 64, 0, h->stream>>>(part, nb, loss);

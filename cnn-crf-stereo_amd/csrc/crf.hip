@@ -24,6 +24,7 @@
 // contiguous floats), backward messages go to a scratch volume and the per-node
 // float64 offsets to LDS.
 #include <cmath>
+#include <vector>
 
 #include "crf_chain.hpp"
 
@@ -55,7 +56,6 @@ __device__ __forceinline__ float rho(int d, float L1, float L2, int delta) {
   return a == 0 ? 0.f : (a < delta ? L1 : L2);
 }
 
-// Per-block float64 partial energy.
 // Loss-augmented unaries of structured-SVM training: out = C - margin [l != t_p]
 // (Hamming loss); pixels without ground truth (t outside [0, D)) keep C.
 __global__ void k_loss_augment(const float* C, const int* __restrict__ t, long HW, int D, float margin, float* out) {
@@ -67,12 +67,18 @@ __global__ void k_loss_augment(const float* C, const int* __restrict__ t, long H
   }
 }
 
+// Per-block float64 partial energy.  With a mask, pixels whose mask label is
+// outside [0, D) (no ground truth) are left out: their unary and every edge
+// with such an endpoint.
 __global__ void k_energy_partial(const float* __restrict__ C, const float* __restrict__ wx,
                                  const float* __restrict__ wy, int H, int W, int D, const int* __restrict__ lab,
-                                 float L1, float L2, int delta, double* __restrict__ part) {
+                                 const int* __restrict__ mask, float L1, float L2, int delta,
+                                 double* __restrict__ part) {
   const long HW = (long)H * W;
+  auto in = [&](long q) { return !mask || (mask[q] >= 0 && mask[q] < D); };
   double acc = 0.0;
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < HW; p += (long)gridDim.x * blockDim.x) {
+    if (!in(p)) continue;
     const int x = (int)(p % W), y = (int)(p / W);
     const int l = lab[p];
     if (l < 0 || l >= D) {  // invalid label: the energy is undefined
@@ -80,8 +86,8 @@ __global__ void k_energy_partial(const float* __restrict__ C, const float* __res
       continue;
     }
     acc += (double)C[p * D + l];
-    if (x + 1 < W) acc += (double)wx[p] * (double)rho(l - lab[p + 1], L1, L2, delta);
-    if (y + 1 < H) acc += (double)wy[p] * (double)rho(l - lab[p + W], L1, L2, delta);
+    if (x + 1 < W && in(p + 1)) acc += (double)wx[p] * (double)rho(l - lab[p + 1], L1, L2, delta);
+    if (y + 1 < H && in(p + W)) acc += (double)wy[p] * (double)rho(l - lab[p + W], L1, L2, delta);
   }
   __shared__ double s[256];
   s[threadIdx.x] = acc;
@@ -99,20 +105,24 @@ __global__ void k_ssvm_grad(const int* __restrict__ xhat, const int* __restrict_
                             double* __restrict__ part) {
   const long HW = (long)H * W;
   double d1 = 0.0, d2 = 0.0;
+  // Pixels whose ground truth x* is outside [0, D) are unlabelled: every edge
+  // with such an endpoint is left out of both energies.
+  auto lab = [&](long q) { return xstar[q] >= 0 && xstar[q] < D; };
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < HW; p += (long)gridDim.x * blockDim.x) {
     const int x = (int)(p % W), y = (int)(p / W);
     const int ls = xstar[p], lh = xhat[p];
+    const bool lp = lab(p);
     float gx = 0.f, gy = 0.f;
     auto lev = [&](int a, int b, int which) {
       const int d = a > b ? a - b : b - a;
       return d == 0 ? 0.0 : (((which == 1) == (d < delta)) ? 1.0 : 0.0);
     };
-    if (x + 1 < W) {
+    if (x + 1 < W && lp && lab(p + 1)) {
       d1 += (double)wx[p] * (lev(ls, xstar[p + 1], 1) - lev(lh, xhat[p + 1], 1));
       d2 += (double)wx[p] * (lev(ls, xstar[p + 1], 2) - lev(lh, xhat[p + 1], 2));
       gx = rho(ls - xstar[p + 1], L1, L2, delta) - rho(lh - xhat[p + 1], L1, L2, delta);
     }
-    if (y + 1 < H) {
+    if (y + 1 < H && lp && lab(p + W)) {
       d1 += (double)wy[p] * (lev(ls, xstar[p + W], 1) - lev(lh, xhat[p + W], 1));
       d2 += (double)wy[p] * (lev(ls, xstar[p + W], 2) - lev(lh, xhat[p + W], 2));
       gy = rho(ls - xstar[p + W], L1, L2, delta) - rho(lh - xhat[p + W], L1, L2, delta);
@@ -137,14 +147,15 @@ __global__ void k_ssvm_grad(const int* __restrict__ xhat, const int* __restrict_
   }
 }
 
-// dC[p][k] = [k == x*_p] - [k == xhat_p], written in full (coalesced).
+// dC[p][k] = [k == x*_p] - [k == xhat_p], written in full (coalesced); 0 at
+// unlabelled pixels (x*_p outside [0, D)).
 __global__ void k_ssvm_unary(const int* __restrict__ xhat, const int* __restrict__ xstar, long HW, int D,
                              float* __restrict__ dC) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= HW * D) return;
   const long p = i / D;
-  const int k = (int)(i % D);
-  dC[i] = (k == xstar[p] ? 1.f : 0.f) - (k == xhat[p] ? 1.f : 0.f);
+  const int k = (int)(i % D), ts = xstar[p];
+  dC[i] = (ts >= 0 && ts < D) ? (k == ts ? 1.f : 0.f) - (k == xhat[p] ? 1.f : 0.f) : 0.f;
 }
 
 __global__ void k_sum_pairs(const double* __restrict__ part, int nb, double* __restrict__ out) {
@@ -182,12 +193,17 @@ extern "C" int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, 
   CCS_CHECK(direction != 1 || (wy && cost), CCS_ERR_ARG, "ccs_slackprop_sweep: columns need cost and wy");
   CCS_CHECK(direction != 2 || (wx && labels), CCS_ERR_ARG, "ccs_slackprop_sweep: decode needs wx and labels");
   const size_t vol = (size_t)H * W * D;
-  if ((rc = h->crf_msg.reserve(vol * sizeof(float)))) return rc;
+  if ((rc = h->crf_msg.reserve(vol * sizeof(float), h->stream))) return rc;
   const int nmax = H > W ? H : W;
-  if ((rc = h->crf_small.reserve(sizeof(double) * 2 * (size_t)nmax))) return rc;
+  if ((rc = h->crf_small.reserve(sizeof(double) * 2 * (size_t)nmax, h->stream))) return rc;
   double* lam = (double*)h->crf_small.ptr;
   double* kept = lam + nmax;
   ChainArgs a{};
+  if (direction == 3) {  // the first sweep of a run reads every cost: it flags NaN per row
+    if ((rc = h->crf_nan.reserve(sizeof(int) * (size_t)H, h->stream))) return rc;
+    a.nan_flags = (int*)h->crf_nan.ptr;
+    h->nan_chains = H;
+  }
   a.cost = cost; a.A = A; a.A_in = direction == 3 ? cost : A;
   a.msg = (float*)h->crf_msg.ptr; a.lam = lam; a.kept = kept;
   a.D = D; a.delta = delta; a.L1 = L1; a.L2 = L2;
@@ -217,12 +233,15 @@ extern "C" int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx,
   if (rc) return rc;
   CCS_CHECK(iters >= 0 && iters <= 10000, CCS_ERR_ARG, "ccs_slackprop: bad iteration count %d", iters);
   const size_t vol = (size_t)H * W * D;
-  if ((rc = h->crf_a.reserve(vol * sizeof(float)))) return rc;
-  if ((rc = h->crf_sums.reserve(sizeof(double) * (4 * (size_t)iters + 2)))) return rc;
+  if ((rc = h->crf_a.reserve(vol * sizeof(float), h->stream))) return rc;
+  if ((rc = h->crf_sums.reserve(sizeof(double) * (4 * (size_t)iters + 2), h->stream))) return rc;
   float* A = (float*)h->crf_a.ptr;
   double* S = (double*)h->crf_sums.ptr;  // S_0 .. S_{2 iters}
   double* K = S + 2 * iters + 1;         // K_0 .. K_{2 iters - 1}
-  if (iters == 0) CCS_HIP(hipMemcpyAsync(A, cost, vol * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+  if (iters == 0) {
+    CCS_HIP(hipMemcpyAsync(A, cost, vol * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    h->nan_chains = 0;  // no row sweep reads the costs
+  }
   for (int it = 0; it < iters; ++it) {
     // The first row sweep reads the unaries from cost (A = cost is never copied).
     if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, it == 0 ? 3 : 0, nullptr, S + 2 * it,
@@ -250,13 +269,30 @@ extern "C" int ccs_slackprop_decode_gap(ccs_handle_t h, const float* unaries, co
   int rc = check_crf_args(H, W, D, L1, L2, delta);
   if (rc) return rc;
   const size_t vol = (size_t)H * W * D;
-  if ((rc = h->crf_msg.reserve(vol * sizeof(float)))) return rc;
-  if ((rc = h->crf_small.reserve(sizeof(double) * 2 * (size_t)H))) return rc;
+  if ((rc = h->crf_msg.reserve(vol * sizeof(float), h->stream))) return rc;
+  if ((rc = h->crf_small.reserve(sizeof(double) * 2 * (size_t)H, h->stream))) return rc;
   ChainArgs dec{};
   dec.A = const_cast<float*>(unaries); dec.A_in = unaries; dec.w = wx; dec.msg = (float*)h->crf_msg.ptr;
   dec.labels = labels; dec.gap = gap; dec.lam = (double*)h->crf_small.ptr; dec.kept = dec.lam + H;
   dec.pcs = W; dec.pns = 1; dec.n_chains = H; dec.n = W; dec.D = D; dec.delta = delta; dec.L1 = L1; dec.L2 = L2;
   return launch_chain(kRowsDecode, dec, h->stream);
+}
+
+extern "C" int ccs_slackprop_status(ccs_handle_t h) {
+  CCS_CHECK(h != nullptr, CCS_ERR_ARG, "ccs_slackprop_status: NULL handle");
+  CCS_HIP(hipStreamSynchronize(h->stream));
+  if (h->nan_chains <= 0) return CCS_OK;
+  std::vector<int> f((size_t)h->nan_chains);
+  CCS_HIP(hipMemcpy(f.data(), h->crf_nan.ptr, sizeof(int) * f.size(), hipMemcpyDeviceToHost));
+  int rows = 0, first = -1;
+  for (size_t i = 0; i < f.size(); ++i)
+    if (f[i]) {
+      if (first < 0) first = (int)i;
+      ++rows;
+    }
+  CCS_CHECK(rows == 0, CCS_ERR_NAN, "slackprop: the cost volume holds NaN in %d row(s), the first at y = %d", rows,
+            first);
+  return CCS_OK;
 }
 
 extern "C" int ccs_slackprop_forward(ccs_handle_t h, const float* cost_volume, const float* wx, const float* wy,
@@ -266,7 +302,7 @@ extern "C" int ccs_slackprop_forward(ccs_handle_t h, const float* cost_volume, c
   CCS_CHECK(h && output, CCS_ERR_ARG, "ccs_slackprop_forward: NULL argument");
   const size_t hw = (size_t)ih * iw;
   int rc;
-  if ((rc = h->misc.reserve(sizeof(int) * hw + sizeof(double)))) return rc;
+  if ((rc = h->misc.reserve(sizeof(int) * hw + sizeof(double), h->stream))) return rc;
   int* lab = (int*)h->misc.ptr;
   double* lb = (double*)((char*)h->misc.ptr + ((sizeof(int) * hw + 7) / 8) * 8);
   if ((rc = ccs_slackprop(h, cost_volume, wx, wy, ih, iw, ic, P1, P2, 2, max_iter, lab, nullptr, nullptr, lb)))
@@ -275,7 +311,27 @@ extern "C" int ccs_slackprop_forward(ccs_handle_t h, const float* cost_volume, c
   if ((rc = ccs_remap(h, lab, ih, iw, 0.f, 1.f, output))) return rc;
   double lbh = 0.0;
   CCS_HIP(hipMemcpyAsync(&lbh, lb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  CCS_HIP(hipStreamSynchronize(h->stream));
+  if ((rc = ccs_slackprop_status(h))) return rc;  // synchronises
+  if (lower_bound) *lower_bound = (float)lbh;
+  return CCS_OK;
+}
+
+extern "C" int ccs_slackprop_forward_volume_out(ccs_handle_t h, const float* cost_volume, const float* wx,
+                                                const float* wy, float* output, int in, int ic, int ih, int iw,
+                                                float P1, float P2, int max_iter, int delta, float* lower_bound) {
+  CCS_CHECK(in == 1, CCS_ERR_UNSUPPORTED, "ccs_slackprop_forward_volume_out: only one sample (in == 1) is supported");
+  CCS_CHECK(h && output, CCS_ERR_ARG, "ccs_slackprop_forward_volume_out: NULL argument");
+  const size_t hw = (size_t)ih * iw;
+  int rc;
+  if ((rc = h->misc.reserve(sizeof(int) * hw + 2 * sizeof(double), h->stream))) return rc;
+  int* lab = (int*)h->misc.ptr;
+  double* lb = (double*)((char*)h->misc.ptr + ((sizeof(int) * hw + 7) / 8) * 8);
+  // output receives the reparametrised volume, the solver's get_C1 (stereo/operators/slackprop.cpp:168).
+  if ((rc = ccs_slackprop(h, cost_volume, wx, wy, ih, iw, ic, P1, P2, delta, max_iter, lab, output, nullptr, lb)))
+    return rc;
+  double lbh = 0.0;
+  CCS_HIP(hipMemcpyAsync(&lbh, lb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if ((rc = ccs_slackprop_status(h))) return rc;  // synchronises
   if (lower_bound) *lower_bound = (float)lbh;
   return CCS_OK;
 }
@@ -291,17 +347,29 @@ extern "C" int ccs_crf_loss_augment(ccs_handle_t h, const float* cost, const int
   return CCS_OK;
 }
 
-extern "C" int ccs_crf_energy(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H, int W,
-                              int D, const int* labels, float L1, float L2, int delta, double* energy) {
-  CCS_CHECK(h && cost && wx && wy && labels && energy, CCS_ERR_ARG, "ccs_crf_energy: NULL argument");
-  const int nb = 256;
+static int crf_energy(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H, int W, int D,
+                      const int* labels, const int* mask, float L1, float L2, int delta, double* energy) {
+  const int nb = kReduceBlocks;
   int rc;
-  if ((rc = h->misc2.reserve(sizeof(double) * nb))) return rc;
+  if ((rc = h->misc2.reserve(sizeof(double) * nb, h->stream))) return rc;
   double* part = (double*)h->misc2.ptr;
-  k_energy_partial<<<nb, 256, 0, h->stream>>>(cost, wx, wy, H, W, D, labels, L1, L2, delta, part);
+  k_energy_partial<<<nb, 256, 0, h->stream>>>(cost, wx, wy, H, W, D, labels, mask, L1, L2, delta, part);
   k_sum_f64<<<1, 256, 0, h->stream>>>(part, nb, energy);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
+}
+
+extern "C" int ccs_crf_energy(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H, int W,
+                              int D, const int* labels, float L1, float L2, int delta, double* energy) {
+  CCS_CHECK(h && cost && wx && wy && labels && energy, CCS_ERR_ARG, "ccs_crf_energy: NULL argument");
+  return crf_energy(h, cost, wx, wy, H, W, D, labels, nullptr, L1, L2, delta, energy);
+}
+
+extern "C" int ccs_crf_energy_masked(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H,
+                                     int W, int D, const int* labels, const int* mask, float L1, float L2, int delta,
+                                     double* energy) {
+  CCS_CHECK(h && cost && wx && wy && labels && mask && energy, CCS_ERR_ARG, "ccs_crf_energy_masked: NULL argument");
+  return crf_energy(h, cost, wx, wy, H, W, D, labels, mask, L1, L2, delta, energy);
 }
 
 extern "C" int ccs_crf_ssvm_grad(ccs_handle_t h, const int* xhat, const int* xstar, const float* wx,
@@ -309,9 +377,9 @@ extern "C" int ccs_crf_ssvm_grad(ccs_handle_t h, const int* xhat, const int* xst
                                  double* dL, float* dwx, float* dwy) {
   CCS_CHECK(h && xhat && xstar && wx && wy && dC && dL, CCS_ERR_ARG, "ccs_crf_ssvm_grad: NULL argument");
   const long HW = (long)H * W;
-  const int nb = 256;
+  const int nb = kReduceBlocks;
   int rc;
-  if ((rc = h->misc2.reserve(sizeof(double) * 2 * nb))) return rc;
+  if ((rc = h->misc2.reserve(sizeof(double) * 2 * nb, h->stream))) return rc;
   double* part = (double*)h->misc2.ptr;
   k_ssvm_unary<<<ceil_div(HW * D, 256), 256, 0, h->stream>>>(xhat, xstar, HW, D, dC);
   k_ssvm_grad<<<nb, 256, 0, h->stream>>>(xhat, xstar, wx, wy, H, W, D, L1, L2, delta, dwx, dwy, part);

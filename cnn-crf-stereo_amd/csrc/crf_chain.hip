@@ -31,7 +31,9 @@
 // the min/max trees then run as fused DPP instructions without NaN-quieting
 // moves.  Costs must be finite or +inf; a NaN cost gives unspecified labels
 // (never an out-of-range memory access: labels are only stored, and every
-// address is computed from chain/node/lane indices).
+// address is computed from chain/node/lane indices).  The row minorant can
+// flag NaN unaries per chain (ChainArgs::nan_flags) with a class test of every
+// value it reads, which the host turns into a status code.
 #include <cstdlib>
 
 #include "crf_chain.hpp"
@@ -137,6 +139,16 @@ __device__ __forceinline__ float local_min(const Vec<LPL>& x) {
 #pragma unroll
   for (int j = 1; j < LPL; ++j) m = fminf(m, x.v[j]);
   return m;
+}
+
+// True in a lane if any of its values is a NaN (a class test, which
+// -fno-honor-nans cannot fold away).
+template <int LPL>
+__device__ __forceinline__ bool any_nan(const Vec<LPL>& x) {
+  bool r = false;
+#pragma unroll
+  for (int j = 0; j < LPL; ++j) r |= __builtin_amdgcn_class(x.v[j], 3);  // signalling or quiet NaN
+  return r;
 }
 
 // The same values in new registers.  The sweeps keep their prefetched nodes in
@@ -403,7 +415,7 @@ __device__ double chain_kept(const float* v, const double* b, int n, int lane, d
 // of registers and the load for node s + P is issued right after slot s is read
 // (clamped to the last node, so it never leaves the chain).  The last partial
 // block is an unrolled tail that issues no loads.
-template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG>
+template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK>
 __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
   constexpr bool kCols = MODE == kColsMinorant;
@@ -487,6 +499,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     }
     u_lds[i] = u;  // every lane writes the same value
   };
+  bool nan_seen = false;  // NANCHK: a unary read by this lane was NaN
   if (nb > 0) {
 #pragma unroll
     for (int j = 0; j < P; ++j) load_b(j, ra[j], rc[j]);
@@ -497,6 +510,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     const float wnext = wlane_b(s0 + P);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
+      if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
       // Slot j is consumed (U is a new value) before it is refilled, so the
       // refill reuses the slot's registers.
       const Vec<LPL> U = plus_bh(theta_of(ra[j], rc[j]));
@@ -508,11 +522,19 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   }
 #pragma unroll
   for (int j = 0; j < P; ++j)
-    if (s0 + j < nb) step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])), lane_weight(wv, j));
+    if (s0 + j < nb) {
+      if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
+      step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])), lane_weight(wv, j));
+    }
   const double bsum = chain_offsets(u_lds, b_lds, n, lane);  // b_0 = sum of all minima
 
   // bh holds Bh[0]; theta[0] is read directly.
   const Vec<LPL> a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
+  if constexpr (NANCHK) {
+    nan_seen |= any_nan(a0);
+    const bool any = __ballot(nan_seen) != 0;  // all lanes take part
+    if (lane == 0) a.nan_flags[c] = any ? 1 : 0;
+  }
   Vec<LPL> c0;
   if constexpr (kCols) c0 = ld_node<LPL, FULL>(Cb, lane, D, kInfF);
   const Vec<LPL> th0 = theta_of(a0, c0);
@@ -884,12 +906,12 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   }
 }
 
-template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG>
+template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK = false>
 static void launch_sweep(ChainArgs& a, size_t lds, hipStream_t st) {
   if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
-    (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC, SEG>,
+    (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  k_chain<LPL, MODE, P, FULL, DC, SEG><<<a.n_chains, 64, lds, st>>>(a);
+  k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK><<<a.n_chains, 64, lds, st>>>(a);
 }
 
 template <int LPL, int MODE, int P, bool FULL, int DC>
@@ -905,6 +927,8 @@ static void launch_one(ChainArgs& a, size_t lds, bool seg, hipStream_t st) {
       launch_sweep<LPL, MODE, P, FULL, DC, true>(a, lds, st);
     else
       launch_sweep<LPL, MODE, P, FULL, DC, false>(a, lds, st);
+  } else if (a.nan_flags) {
+    launch_sweep<LPL, MODE, P, FULL, DC, false, true>(a, lds, st);
   } else {
     launch_sweep<LPL, MODE, P, FULL, DC, false>(a, lds, st);
   }

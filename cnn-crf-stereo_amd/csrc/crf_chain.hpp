@@ -20,6 +20,7 @@ struct ChainArgs {
   float* gap;          // decode: second best - best min-marginal (optional)
   double* lam;         // per chain: minimum chain energy
   double* kept;        // per chain: sum of kept constants
+  int* nan_flags;      // row minorant (optional): per chain, 1 if a unary read was NaN, else 0
   long pcs, pns;       // pixel index = chain * pcs + node * pns
   int n_chains, n, D, delta;
   float L1, L2;

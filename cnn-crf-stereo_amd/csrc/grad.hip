@@ -19,9 +19,10 @@ __global__ void k_dz(const float* __restrict__ y, const float* __restrict__ dy, 
   dz[i] = act ? dy[i] * (1.f - yv * yv) : dy[i];
 }
 
-// Partial bias gradient: block b sums pixels [b*chunk, (b+1)*chunk) per channel.
+// Partial bias gradient: block (b, j) sums pixels [b*chunk, (b+1)*chunk) of
+// channels j*256 .. j*256+255.
 __global__ void k_db_partial(const float* __restrict__ dz, long P, int Cs, int chunk, float* __restrict__ part) {
-  const int c = threadIdx.x;
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
   if (c >= Cs) return;
   const long p0 = (long)blockIdx.x * chunk;
   const long p1 = p0 + chunk < P ? p0 + chunk : P;
@@ -122,7 +123,7 @@ extern "C" int ccs_conv2d_backward(ccs_handle_t h, const float* x, int H, int W,
   const size_t part_n = (size_t)nb * (wcount > (size_t)Csk ? wcount : (size_t)Csk);
   const size_t pad_n = dx ? (size_t)Hp * Wp * Csk : 0;
   const size_t need = sizeof(float) * (nz + part_n + pad_n + wcount);
-  int rc = h->misc2.reserve(need);
+  int rc = h->misc2.reserve(need, h->stream);
   if (rc) return rc;
   float* z = (float*)h->misc2.ptr;
   float* part = z + nz;
@@ -132,7 +133,7 @@ extern "C" int ccs_conv2d_backward(ccs_handle_t h, const float* x, int H, int W,
   if ((rc = ccs_conv2d_forward(h, x, CCS_LAYOUT_NHWC, H, W, C, w, b, K, KH, KW, act, CCS_FP32, z))) return rc;
   k_dz<<<ceil_div(nz, 256), 256, 0, h->stream>>>(z, dy, nz, act, z);
   if (db) {
-    k_db_partial<<<nb, Csk <= 1024 ? ((Csk + 63) / 64) * 64 : 1024, 0, h->stream>>>(z, P, Csk, (int)chunk, part);
+    k_db_partial<<<dim3(nb, ceil_div(Csk, 256)), 256, 0, h->stream>>>(z, P, Csk, (int)chunk, part);
     k_reduce_rows<<<ceil_div(Csk, 256), 256, 0, h->stream>>>(part, nb, Csk, part + (size_t)nb * Csk);
     CCS_HIP(hipMemcpyAsync(db, part + (size_t)nb * Csk, sizeof(float) * K, hipMemcpyDeviceToDevice, h->stream));
   }

@@ -23,28 +23,38 @@ int fail(int status, const char* fmt, ...) {
   return status;
 }
 
-int Scratch::reserve(size_t need) {
+int Scratch::reserve(size_t need, hipStream_t st) {
   if (need <= bytes) return CCS_OK;
+  // Growing is stream-ordered (the old buffer is released after the work
+  // queued on st that may still read it), so it blocks no other stream; it
+  // cannot be recorded into a graph, which the caller learns here rather than
+  // from a failed capture.
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(st, &cs);
+  if (e != hipSuccess) return fail(CCS_ERR_HIP, "hipStreamIsCapturing: %s", hipGetErrorString(e));
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(CCS_ERR_ARG,
+                "the handle's scratch must grow to %zu bytes while its stream is being captured: "
+                "call ccs_reserve (or the same entry point once, uncaptured) at this size before capture",
+                need);
   if (ptr) {
-    // The buffer may still be read by queued work of this handle.
-    hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) return fail(CCS_ERR_HIP, "hipDeviceSynchronize: %s", hipGetErrorString(e));
-    (void)hipFree(ptr);
+    e = hipFreeAsync(ptr, st);
+    if (e != hipSuccess) return fail(CCS_ERR_HIP, "hipFreeAsync: %s", hipGetErrorString(e));
     ptr = nullptr;
     bytes = 0;
   }
   size_t rounded = (need + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
-  hipError_t e = hipMalloc(&ptr, rounded);
+  e = hipMallocAsync(&ptr, rounded, st);
   if (e != hipSuccess) {
     ptr = nullptr;
-    return fail(CCS_ERR_ALLOC, "hipMalloc(%zu) failed: %s", rounded, hipGetErrorString(e));
+    return fail(CCS_ERR_ALLOC, "hipMallocAsync(%zu) failed: %s", rounded, hipGetErrorString(e));
   }
   bytes = rounded;
   return CCS_OK;
 }
 
-void Scratch::release() {
-  if (ptr) (void)hipFree(ptr);
+void Scratch::release(hipStream_t st) {
+  if (ptr) (void)hipFreeAsync(ptr, st);
   ptr = nullptr;
   bytes = 0;
 }
@@ -70,20 +80,18 @@ extern "C" int ccs_create(ccs_handle_t* handle, void* hip_stream) {
 
 extern "C" int ccs_destroy(ccs_handle_t h) {
   if (!h) return CCS_OK;
+  for (ccs::Scratch* s : {&h->conv_w, &h->crf_a, &h->crf_msg, &h->crf_small, &h->crf_sums, &h->misc, &h->misc2, &h->crf_nan})
+    s->release(h->stream);
   (void)hipStreamSynchronize(h->stream);
-  h->conv_w.release();
-  h->crf_a.release();
-  h->crf_msg.release();
-  h->crf_small.release();
-  h->crf_sums.release();
-  h->misc.release();
-  h->misc2.release();
   delete h;
   return CCS_OK;
 }
 
 extern "C" int ccs_set_stream(ccs_handle_t h, void* hip_stream) {
   CCS_CHECK(h != nullptr, CCS_ERR_ARG, "ccs_set_stream: NULL handle");
+  // Work queued on the old stream may still use the scratch, which from now on
+  // is released and reused in the order of the new stream.
+  if ((hipStream_t)hip_stream != h->stream) CCS_HIP(hipStreamSynchronize(h->stream));
   h->stream = (hipStream_t)hip_stream;
   return CCS_OK;
 }
@@ -91,5 +99,33 @@ extern "C" int ccs_set_stream(ccs_handle_t h, void* hip_stream) {
 extern "C" int ccs_synchronize(ccs_handle_t h) {
   CCS_CHECK(h != nullptr, CCS_ERR_ARG, "ccs_synchronize: NULL handle");
   CCS_HIP(hipStreamSynchronize(h->stream));
+  return CCS_OK;
+}
+
+extern "C" int ccs_reserve(ccs_handle_t h, int H, int W, int D, int C, int iters) {
+  CCS_CHECK(h != nullptr, CCS_ERR_ARG, "ccs_reserve: NULL handle");
+  CCS_CHECK(H >= 1 && W >= 1 && D >= 1 && C >= 1 && iters >= 0, CCS_ERR_ARG, "ccs_reserve: bad size");
+  const size_t hw = (size_t)H * W, vol = hw * D;
+  const size_t big = (size_t)(H > W ? H : W);
+  auto mx = [](size_t a, size_t b) { return a > b ? a : b; };
+  // Packed weights of the C -> C and 64 -> 64 layers, either precision.
+  size_t packed = 0;
+  for (int p : {CCS_FP32, CCS_BF16})
+    for (int k : {C, 64}) packed = mx(packed, ccs_conv_packed_bytes(k, k, 3, 3, p));
+  // misc: disparity list (correlation statics), image moments (up to 5
+  // channels), labels + bound of the synchronising SlackProp statics.
+  const size_t misc = mx(mx(sizeof(int) * (size_t)D, sizeof(double) * (2 * 5 * (size_t)ccs::kMomentBlocks + 10)),
+                         sizeof(int) * hw + 2 * sizeof(double));
+  // misc2: Gaussian temporary of the edge weights, float64 reduction partials.
+  const size_t misc2 = mx(sizeof(float) * hw, 2 * sizeof(double) * (size_t)ccs::kReduceBlocks);
+  int rc;
+  if (packed && (rc = h->conv_w.reserve(packed, h->stream))) return rc;
+  if ((rc = h->misc.reserve(misc, h->stream))) return rc;
+  if ((rc = h->misc2.reserve(misc2, h->stream))) return rc;
+  if ((rc = h->crf_a.reserve(vol * sizeof(float), h->stream))) return rc;
+  if ((rc = h->crf_msg.reserve(vol * sizeof(float), h->stream))) return rc;
+  if ((rc = h->crf_small.reserve(sizeof(double) * 2 * big, h->stream))) return rc;
+  if ((rc = h->crf_nan.reserve(sizeof(int) * (size_t)H, h->stream))) return rc;
+  if ((rc = h->crf_sums.reserve(sizeof(double) * (4 * (size_t)iters + 2), h->stream))) return rc;
   return CCS_OK;
 }

@@ -15,7 +15,6 @@ __device__ __forceinline__ int reflect(int i, int n) {
   return i < n ? i : period - 1 - i;
 }
 
-constexpr int kMomentBlocks = 128;
 
 // Pass 1: out[c][y][x] = img[c][reflect(y-p)][reflect(x-p)] (stereo/stereo.cu:879-889)
 // and per-block partial sums of the padded values and their squares in float64.
@@ -223,7 +222,7 @@ extern "C" int ccs_pad_normalize(ccs_handle_t h, const float* img, int C, int H,
   const int Ho = H + 2 * pad, Wo = W + 2 * pad;
   const long n = (long)Ho * Wo;
   const int nb = kMomentBlocks;
-  int rc = h->misc.reserve(sizeof(double) * (2 * (size_t)C * nb + 2 * C));
+  int rc = h->misc.reserve(sizeof(double) * (2 * (size_t)C * nb + 2 * C), h->stream);
   if (rc) return rc;
   double* part = (double*)h->misc.ptr;
   double* st = stats ? stats : part + 2 * (size_t)C * nb;
@@ -255,7 +254,7 @@ extern "C" int ccs_edge_weights(ccs_handle_t h, const float* gray, int H, int W,
     s += k[i + t.r];
   }
   for (int i = 0; i < 2 * t.r + 1; ++i) t.k[i] = (float)(k[i] / s);
-  int rc = h->misc2.reserve(sizeof(float) * (size_t)H * W);
+  int rc = h->misc2.reserve(sizeof(float) * (size_t)H * W, h->stream);
   if (rc) return rc;
   float* tmp = (float*)h->misc2.ptr;
   k_gauss_h<<<dim3(ceil_div(W, 256), H), 256, 0, h->stream>>>(gray, H, W, t, tmp);

@@ -24,6 +24,25 @@ namespace {
 
 int prec_code(Precision p) { return p == Precision::BF16 ? CCS_BF16 : CCS_FP32; }
 
+// One convolution of the networks: K filters over C channels, KH x KW, with or
+// without a bias (and tanh) after it.
+struct LayerSpec {
+  int K, C, KH, KW;
+  bool bias;
+};
+
+// Tower: layer 1 3x3, ic -> 100; layers 2..numLayers 2x2, 100 -> 100
+// (stereo/stereonet.cpp:65-132).
+std::vector<LayerSpec> tower_spec(int numLayers, int ic) {
+  std::vector<LayerSpec> v;
+  for (int l = 0; l < numLayers; ++l) v.push_back(l == 0 ? LayerSpec{100, ic, 3, 3, true} : LayerSpec{100, 100, 2, 2, true});
+  return v;
+}
+
+// Pairwise network on the RGB input (stereo/stereonet.cpp:143-168): 3x3 3 -> 64,
+// 3x3 64 -> 64, each with bias + tanh, then the 1x1 64 -> 2 head without bias.
+std::vector<LayerSpec> pairwise_spec() { return {{64, 3, 3, 3, true}, {64, 64, 3, 3, true}, {2, 64, 1, 1, false}}; }
+
 int log2i(int num) {  // floor(log2(num)) for num >= 1, 0 for num <= 1 (stereo/stereo.cpp:56-62)
   int r = 0;
   while (num >>= 1) ++r;
@@ -55,14 +74,15 @@ StereoNet::~StereoNet() {
 void StereoNet::initNet(float min_disp, float max_disp, float step, int rect_corr, Tensor* d_unaryOut,
                         Tensor* d_pairwiseOut) {
   if (m_initialized) arg_fail("StereoNet::initNet: already initialised");
+  const std::vector<LayerSpec> spec = tower_spec(m_numLayers, m_ic);
   for (std::vector<Operator*>* ops : {&m_leftOps, &m_rightOps}) {
-    // layer 1: 3x3, ic -> 100; layers 2..numLayers: 2x2, 100 -> 100 (stereo/stereonet.cpp:65-132)
-    for (int l = 0; l < m_numLayers; ++l) {
-      Convolution* conv = l == 0 ? new Convolution(Convolution::CONVOLUTION, m_in, m_ic, m_ih, m_iw, 100, m_ic, 3, 3)
-                                 : new Convolution(Convolution::CONVOLUTION, ops->back()->outTensorDesc(), 100, 100, 2, 2);
+    for (size_t l = 0; l < spec.size(); ++l) {
+      const LayerSpec& L = spec[l];
+      Convolution* conv = l == 0 ? new Convolution(Convolution::CONVOLUTION, m_in, m_ic, m_ih, m_iw, L.K, L.C, L.KH, L.KW)
+                                 : new Convolution(Convolution::CONVOLUTION, ops->back()->outTensorDesc(), L.K, L.C, L.KH, L.KW);
       conv->setPrecision(m_precision);
       ops->push_back(conv);
-      Bias* bias = new Bias(conv->outTensorDesc(), 1, 100, 1, 1);
+      Bias* bias = new Bias(conv->outTensorDesc(), 1, L.K, 1, 1);
       ops->push_back(bias);
       ops->push_back(new Activation(conv->outTensorDesc(), Activation::TANH));
       conv->setFusedEpilogue(bias->getParams(), true);
@@ -95,23 +115,23 @@ void StereoNet::initPairwise(Tensor* d_pairwiseOut) {
       od.precision != Precision::FP32)
     arg_fail("StereoNet::initNet: d_pairwiseOut must be a 1 x 2 x " + std::to_string(ih - 4) + " x " +
              std::to_string(iw - 4) + " float32 planar tensor");
-  auto* c1 = new Convolution(Convolution::CONVOLUTION, m_in, 3, ih, iw, 64, 3, 3, 3);
-  m_pairwiseOps.push_back(c1);
-  c1->setPrecision(m_precision);
-  auto* b1 = new Bias(c1->outTensorDesc(), 1, 64, 1, 1);
-  m_pairwiseOps.push_back(b1);
-  m_pairwiseOps.push_back(new Activation(c1->outTensorDesc(), Activation::TANH));
-  c1->setFusedEpilogue(b1->getParams(), true);
-  auto* c2 = new Convolution(Convolution::CONVOLUTION, c1->outTensorDesc(), 64, 64, 3, 3);
-  m_pairwiseOps.push_back(c2);
-  c2->setPrecision(m_precision);
-  auto* b2 = new Bias(c2->outTensorDesc(), 1, 64, 1, 1);
-  m_pairwiseOps.push_back(b2);
-  m_pairwiseOps.push_back(new Activation(c2->outTensorDesc(), Activation::TANH));
-  c2->setFusedEpilogue(b2->getParams(), true);
-  auto* c3 = new Convolution(Convolution::CONVOLUTION, c2->outTensorDesc(), 2, 64, 1, 1);
-  m_pairwiseOps.push_back(c3);
-  c3->deleteOutputMemory();  // the head kernel writes d_pairwiseOut directly
+  const std::vector<LayerSpec> spec = pairwise_spec();
+  for (size_t l = 0; l < spec.size(); ++l) {
+    const LayerSpec& L = spec[l];
+    auto* c = l == 0 ? new Convolution(Convolution::CONVOLUTION, m_in, L.C, ih, iw, L.K, L.C, L.KH, L.KW)
+                     : new Convolution(Convolution::CONVOLUTION, m_pairwiseOps.back()->outTensorDesc(), L.K, L.C,
+                                       L.KH, L.KW);
+    m_pairwiseOps.push_back(c);
+    if (!L.bias) {
+      c->deleteOutputMemory();  // the head kernel writes d_pairwiseOut directly
+      continue;
+    }
+    c->setPrecision(m_precision);
+    auto* b = new Bias(c->outTensorDesc(), 1, L.K, 1, 1);
+    m_pairwiseOps.push_back(b);
+    m_pairwiseOps.push_back(new Activation(c->outTensorDesc(), Activation::TANH));
+    c->setFusedEpilogue(b->getParams(), true);
+  }
   m_pairwiseOut = d_pairwiseOut;
 }
 
@@ -148,6 +168,44 @@ void StereoNet::setAllParams(const std::string& npzPath) {
     ++param_idx;
   }
   // Trailing arrays (the learned L1, L2 of CRF models) are not network parameters.
+}
+
+std::vector<TensorDesc> StereoNet::parameterShapes(int numLayers, int ic, bool pairwise) {
+  if (numLayers != 3 && numLayers != 5 && numLayers != 7)
+    arg_fail("StereoNet: numLayers must be 3, 5 or 7 (stereo/stereo.cpp:198-202)");
+  std::vector<TensorDesc> v;
+  auto add = [&](const std::vector<LayerSpec>& spec) {
+    for (const LayerSpec& L : spec) {
+      v.push_back(TensorDesc(L.K, L.C, L.KH, L.KW));
+      if (L.bias) v.push_back(TensorDesc(1, L.K, 1, 1));
+    }
+  };
+  add(tower_spec(numLayers, ic));
+  if (pairwise) add(pairwise_spec());
+  return v;
+}
+
+std::vector<std::string> StereoNet::checkParameterFile(const std::string& npzPath, int numLayers, int ic,
+                                                       bool pairwise) {
+  const std::map<std::string, HostArray> all = npz::load(npzPath);
+  const std::vector<TensorDesc> shapes = parameterShapes(numLayers, ic, pairwise);
+  for (size_t i = 0; i < shapes.size(); ++i) {
+    const std::string key = "arr_" + std::to_string(i);
+    auto it = all.find(key);
+    if (it == all.end()) throw Error(CCS_ERR_IO, npzPath + ": no array '" + key + "'");
+    try {
+      Operator::checkParams(it->second, shapes[i]);
+    } catch (const Error& e) {
+      throw Error(e.status(), npzPath + ": " + key + ": " + e.what());
+    }
+  }
+  std::vector<std::string> unread;
+  for (size_t i = shapes.size();; ++i) {
+    const std::string key = "arr_" + std::to_string(i);
+    if (!all.count(key)) break;
+    unread.push_back(key);
+  }
+  return unread;
 }
 
 void StereoNet::setDisparities(float min_disp, float max_disp, float step, Tensor* d_out) {

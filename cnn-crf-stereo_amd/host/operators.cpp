@@ -55,10 +55,7 @@ void Operator::setParams(const std::string& npzPath, const std::string& array) {
   setParams(npz::load_array(npzPath, array));
 }
 
-void Operator::setParams(const HostArray& h_params) {
-  Tensor* p = getParams();
-  if (!p) arg_fail("setParams: operator has no parameters");
-  const TensorDesc& d = p->desc();
+void Operator::checkParams(const HostArray& h_params, const TensorDesc& d) {
   const size_t want = (size_t)d.n * d.c * d.h * d.w;
   if (h_params.shape.size() != 1 && h_params.shape.size() != 4)
     arg_fail("setParams: only 1-D and 4-D parameter arrays are accepted");
@@ -68,6 +65,12 @@ void Operator::setParams(const HostArray& h_params) {
   if (h_params.shape.size() == 4 &&
       (h_params.shape[0] != d.n || h_params.shape[1] != d.c || h_params.shape[2] != d.h || h_params.shape[3] != d.w))
     arg_fail("setParams: 4-D parameter shape does not match the operator");
+}
+
+void Operator::setParams(const HostArray& h_params) {
+  Tensor* p = getParams();
+  if (!p) arg_fail("setParams: operator has no parameters");
+  checkParams(h_params, p->desc());
   p->copyFromHost(h_params.data.data());
   paramsChanged();
 }
@@ -391,7 +394,7 @@ Tensor* SlackProp::forward(std::vector<Tensor*>& d_inputs, Handle& handle) {
         "SlackProp::forward");
   check(ccs_remap(handle.get(), (const int*)labels.raw(), d.h, d.w, 0.f, 1.f, m_output->data()),
         "SlackProp::forward: labels");
-  handle.synchronize();
+  check(ccs_slackprop_status(handle.get()), "SlackProp::forward");  // synchronises; NaN costs throw
   double v = 0;
   if (hipMemcpy(&v, lb.raw(), sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     throw Error(CCS_ERR_HIP, "SlackProp::forward: reading the bound");
@@ -410,18 +413,12 @@ float SlackProp::forward(float* costVolume, float* wx, float* wy, float* output,
 
 float SlackProp::forwardVolumeOut(float* costVolume, float* wx, float* wy, float* output, int in, int ic, int ih,
                                   int iw, float P1, float P2, int max_iter, int delta, Handle* handle) {
-  if (in != 1) throw Error(CCS_ERR_UNSUPPORTED, "SlackProp::forwardVolumeOut: N=1 only");
   Handle& h = handle ? *handle : default_handle();
-  Tensor labels(TensorDesc(1, 1, ih, iw));
-  Tensor lb(TensorDesc(1, 1, 1, 2));
-  check(ccs_slackprop(h.get(), costVolume, wx, wy, ih, iw, ic, P1, P2, delta, max_iter, (int*)labels.raw(), output,
-                      nullptr, (double*)lb.raw()),
+  float lb = 0.f;
+  check(ccs_slackprop_forward_volume_out(h.get(), costVolume, wx, wy, output, in, ic, ih, iw, P1, P2, max_iter, delta,
+                                         &lb),
         "SlackProp::forwardVolumeOut");
-  h.synchronize();
-  double v = 0;
-  if (hipMemcpy(&v, lb.raw(), sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-    throw Error(CCS_ERR_HIP, "SlackProp::forwardVolumeOut: reading the bound");
-  return (float)v;
+  return lb;
 }
 
 }  // namespace ccs

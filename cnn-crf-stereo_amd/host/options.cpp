@@ -43,7 +43,7 @@ const Spec kSpecs[] = {
     {"crf-iterations", "4", "Parameter for CRF inference"},
     {"ref-iterations", "10", "Refinement iterations"},
     {"warps", "2", "Refinement warps"},
-    {"precision", "bf16", "Feature CNN arithmetic on MI355X: bf16 (MFMA) or fp32"},
+    {"precision", "fp32", "Feature CNN arithmetic on MI355X: fp32 (as the reference) or bf16 (MFMA, faster)"},
     {"cnn-gc", "true", "Free each feature map once consumed (false: keep them and run the two towers on two streams)"},
 };
 

@@ -24,7 +24,7 @@ static void dump(const std::string& key, const HostArray& a) {
 
 static int run(int argc, char** argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: ccs_selftest npz-dump|npy-dump|npz-write|npy-write|options|labels|imread|pfm-write|png-write ...\n");
+    std::fprintf(stderr, "usage: ccs_selftest npz-dump|npy-dump|npz-write|npy-write|options|config|params|labels|imread|pfm-write|png-write ...\n");
     return 1;
   }
   const std::string cmd = argv[1];
@@ -49,6 +49,20 @@ static int run(int argc, char** argv) {
                           "L1", "L2", "delta", "alpha", "beta", "crf-iterations", "num-layers", "output-file",
                           "gpu", "parameter-file", "precision"})
       std::printf("%s=%s\n", k, o.has(k) ? o.str(k).c_str() : "<unset>");
+  } else if (cmd == "config" && argc == 3) {
+    // A model .cfg through Options::parseFile, as stereo_img --config-file reads it.
+    Options o;
+    o.parseFile(argv[2]);
+    for (const char* k : {"matching", "num-layers", "inference", "lambda", "L1", "L2", "delta", "alpha", "beta",
+                          "disp-min", "disp-step", "disp-max", "crf-iterations", "with-xy"})
+      std::printf("%s=%s\n", k, o.has(k) ? o.str(k).c_str() : "<unset>");
+  } else if (cmd == "params" && argc == 6) {
+    // StereoNet's parameter walk over an npz file, host only: numLayers, input
+    // channels, pairwise network (0/1).
+    const auto unread = StereoNet::checkParameterFile(argv[2], atoi(argv[3]), atoi(argv[4]), atoi(argv[5]) != 0);
+    std::printf("ok %zu", StereoNet::parameterShapes(atoi(argv[3]), atoi(argv[4]), atoi(argv[5]) != 0).size());
+    for (const auto& k : unread) std::printf(" %s", k.c_str());
+    std::printf("\n");
   } else if (cmd == "labels" && argc == 5) {
     float mx = 0;
     const int n = label_count((float)atof(argv[2]), (float)atof(argv[3]), (float)atof(argv[4]), &mx);

@@ -209,11 +209,32 @@ def slackprop_forward_static(h, cost, wx, wy, P1, P2, max_iter=5):
     return out, lb.value
 
 
-def crf_energy(h, cost, wx, wy, labels, L1, L2, delta):
+def slackprop_forward_volume_out(h, cost, wx, wy, P1, P2, max_iter, delta):
+    """The reference's SlackProp::forwardVolumeOut: the reparametrised volume and the bound."""
+    H, W, D = cost.shape
+    out = torch.empty_like(cost)
+    lb = ctypes.c_float()
+    call("ccs_slackprop_forward_volume_out", h.h, ptr(cost), ptr(wx), ptr(wy), ptr(out), 1, D, H, W, float(P1),
+         float(P2), int(max_iter), int(delta), ctypes.byref(lb))
+    return out, lb.value
+
+
+def slackprop_status(h):
+    """Raises CCSError (status ERR_NAN) if the last CRF run on h read a NaN cost."""
+    call("ccs_slackprop_status", h.h)
+
+
+def crf_energy(h, cost, wx, wy, labels, L1, L2, delta, mask=None):
+    """CRF energy of labels (float64 on the device); with mask, only over the
+    pixels whose mask label lies in [0, D)."""
     H, W, D = cost.shape
     e = torch.zeros(1, dtype=torch.float64, device="cuda")
-    call("ccs_crf_energy", h.h, ptr(cost), ptr(wx), ptr(wy), H, W, D, ptr(labels), float(L1), float(L2),
-         int(delta), ptr(e))
+    if mask is None:
+        call("ccs_crf_energy", h.h, ptr(cost), ptr(wx), ptr(wy), H, W, D, ptr(labels), float(L1), float(L2),
+             int(delta), ptr(e))
+    else:
+        call("ccs_crf_energy_masked", h.h, ptr(cost), ptr(wx), ptr(wy), H, W, D, ptr(labels), ptr(mask),
+             float(L1), float(L2), int(delta), ptr(e))
     return e
 
 

@@ -3,9 +3,12 @@
  *
  * Plain C: pointers, sizes and status codes only.  Every device pointer is
  * caller-owned HBM; work is enqueued on the handle's HIP stream and nothing
- * blocks unless a function says so.  Functions return CCS_OK (0) or a
- * negative status; ccs_last_error() gives the message of the calling thread's
- * last failure.  Each entry point names the reference interface it replaces
+ * blocks unless a function says so.  A handle's scratch memory grows on first
+ * use at a size, in the order of its stream (no host wait); a call that would
+ * grow it while the stream is being captured into a HIP graph returns
+ * CCS_ERR_ARG instead, so warm a handle up (ccs_reserve) before capturing.
+ * Functions return CCS_OK (0) or a negative status; ccs_last_error() gives the
+ * message of the calling thread's last failure.  Each entry point names the reference interface it replaces
  * (paths relative to the VLOGroup/cnn-crf-stereo source tree).
  *
  * Layouts
@@ -34,7 +37,8 @@ enum {
   CCS_ERR_HIP = -3,         /* HIP runtime error                                */
   CCS_ERR_ALLOC = -4,       /* device allocation failed                         */
   CCS_ERR_IO = -5,          /* file missing or malformed                        */
-  CCS_ERR_SHAPE = -6        /* tensor shape does not match the operator         */
+  CCS_ERR_SHAPE = -6,       /* tensor shape does not match the operator         */
+  CCS_ERR_NAN = -7          /* an input holds NaN where the operator needs numbers */
 };
 
 enum { CCS_LAYOUT_NCHW = 0, CCS_LAYOUT_NHWC = 1 };    /* iu::TensorGpu_32f::MemoryLayout */
@@ -49,8 +53,17 @@ const char* ccs_version(void);
  * stream == NULL uses the null stream. */
 int ccs_create(ccs_handle_t* handle, void* hip_stream);
 int ccs_destroy(ccs_handle_t handle);
+/* Switching streams waits for the work queued on the old one (blocking). */
 int ccs_set_stream(ccs_handle_t handle, void* hip_stream);
 int ccs_synchronize(ccs_handle_t handle);
+
+/* Grows the handle's scratch to what the forward entry points of this header
+ * need for an H x W frame with D labels, C feature channels (and 64-channel
+ * pairwise layers) and `iters` CRF iterations, so that later calls at or below
+ * those sizes allocate nothing and may be captured into a HIP graph.  The
+ * adjoints (ccs_conv2d_backward) are not covered.  Stream-ordered.  No
+ * reference counterpart (cudnnHandle_t workspaces were sized per call). */
+int ccs_reserve(ccs_handle_t handle, int H, int W, int D, int C, int iters);
 
 /* Channel stride of a feature map with C channels (C rounded up to a multiple of 8). */
 int ccs_feature_stride(int C);
@@ -155,10 +168,20 @@ int ccs_edge_weights(ccs_handle_t h, const float* gray, int H, int W, float sigm
 /* SlackProp::forward static, old interface (stereo/operators/slackprop.h:43-44,
  * slackprop.cpp:111-139): same argument order; delta = 2 as in the reference.
  * cost_volume: ih x iw x ic (ic = number of labels), output: ih x iw labels
- * written as float.  *lower_bound receives the dual bound.  Synchronises. */
+ * written as float.  *lower_bound receives the dual bound.  Synchronises;
+ * returns CCS_ERR_NAN when the cost volume holds NaN (max_iter >= 1). */
 int ccs_slackprop_forward(ccs_handle_t h, const float* cost_volume, const float* wx,
                           const float* wy, float* output, int in, int ic, int ih, int iw,
                           float P1, float P2, int max_iter, float* lower_bound);
+
+/* SlackProp::forwardVolumeOut static (stereo/operators/slackprop.h:46-47,
+ * slackprop.cpp:142-173): same argument order; output (ih x iw x ic) receives
+ * the reparametrised cost volume (the solver's get_C1: the final row unaries
+ * of ccs_slackprop) instead of labels, delta is the caller's.  Synchronises;
+ * CCS_ERR_NAN as ccs_slackprop_forward. */
+int ccs_slackprop_forward_volume_out(ccs_handle_t h, const float* cost_volume, const float* wx,
+                                     const float* wy, float* output, int in, int ic, int ih, int iw,
+                                     float P1, float P2, int max_iter, int delta, float* lower_bound);
 
 /* SlackProp CRF inference (the restated dual minorize-maximize over row and column
  * chains, float32 min-sum, float64 bound bookkeeping).  Call sites it replaces:
@@ -169,10 +192,18 @@ int ccs_slackprop_forward(ccs_handle_t h, const float* cost_volume, const float*
  * 4*iters+1 doubles) receives the per-half-step chain minima S_0..S_{2 iters}
  * followed by the kept constants K_0..K_{2 iters - 1}; the bound after half-step
  * j >= 1 is K_{j-1} + S_j.  lower_bound (optional, device double) receives the
- * final bound.  Stream-ordered, no host synchronisation. */
+ * final bound.  Stream-ordered, no host synchronisation: ccs_slackprop_status
+ * reports a NaN cost afterwards. */
 int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H,
                   int W, int D, float L1, float L2, int delta, int iters, int* labels,
                   float* unaries_out, double* sums, double* lower_bound);
+
+/* Waits for the handle's stream and returns CCS_ERR_NAN if the first row sweep
+ * of the last ccs_slackprop run on this handle (or the last direction-3
+ * ccs_slackprop_sweep) read a NaN cost, CCS_OK otherwise (also when no such
+ * sweep ran: iters == 0).  The reference has no such check
+ * (stereo/error_util.h:23-52 covers CUDA and cuDNN status codes only). */
+int ccs_slackprop_status(ccs_handle_t h);
 
 /* One half-step of ccs_slackprop on caller-held row unaries A (H x W x D, set to
  * a copy of cost before the first call): direction 0 sweeps the rows
@@ -198,6 +229,14 @@ int ccs_crf_energy(ccs_handle_t h, const float* cost, const float* wx, const flo
                    int W, int D, const int* labels, float L1, float L2, int delta,
                    double* energy);
 
+/* ccs_crf_energy over the labelled part of the grid: pixels p whose mask[p]
+ * (e.g. the ground truth of training) lies outside [0, D) are left out, with
+ * their unary and every edge that has such an endpoint.  The energies of
+ * ccs_crf_ssvm_grad.  No reference counterpart (training lived outside the repo). */
+int ccs_crf_energy_masked(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H,
+                          int W, int D, const int* labels, const int* mask, float L1, float L2,
+                          int delta, double* energy);
+
 /* Loss-augmented unaries for structured-SVM training: out = cost - margin [l != target]
  * per pixel (Hamming loss; targets outside [0, D) leave the pixel unchanged);
  * out may alias cost.  Running ccs_slackprop on out gives the most violating
@@ -207,7 +246,10 @@ int ccs_crf_loss_augment(ccs_handle_t h, const float* cost, const int* target, l
 
 /* Structured-SVM subgradient of E(x*) - E(xhat) w.r.t. the unaries (dC, H x W x D,
  * overwritten), L1/L2 (dL, device, 2 doubles) and the edge weights (dwx, dwy,
- * H x W, optional).  No reference counterpart (training lived outside the repo). */
+ * H x W, optional).  Pixels whose x* lies outside [0, D) are unlabelled: both
+ * energies leave out their unary and their edges (ccs_crf_energy_masked with
+ * mask = x*), so dC is 0 there.  No reference counterpart (training lived
+ * outside the repo). */
 int ccs_crf_ssvm_grad(ccs_handle_t h, const int* xhat, const int* xstar, const float* wx,
                       const float* wy, int H, int W, int D, float L1, float L2, int delta,
                       float* dC, double* dL, float* dwx, float* dwy);

@@ -158,6 +158,9 @@ class Operator {
   // (stereo/operators/operator.cpp:55-85).
   void setParams(const std::string& npzPath, const std::string& array);
   void setParams(const HostArray& h_params);
+  // The checks of setParams for a parameter of shape `want` (host only):
+  // throws ccs::Error when h_params does not fit.
+  static void checkParams(const HostArray& h_params, const TensorDesc& want);
   Type type() const { return m_type; }
 
  protected:
@@ -331,6 +334,17 @@ class StereoNet {
 
   void setDisparities(float min_disp, float max_disp, float step, Tensor* d_out = nullptr);
   void setAllParams(const std::string& npzPath);
+  // Host-only forms of the parameter walk.  parameterShapes lists, in the order
+  // of the arr_<i> keys, the parameters setAllParams loads for this
+  // architecture: per layer the filter (K x C x KH x KW) and the bias (1 x K x
+  // 1 x 1), the tower's layers first, then the pairwise network's (whose 1x1
+  // head has no bias).  checkParameterFile walks an npz file against it with
+  // setParams's checks, touching no device; it returns the keys of the arrays
+  // left unread (the trailing learned L1, L2 of CRF models) and throws
+  // ccs::Error where setAllParams would.
+  static std::vector<TensorDesc> parameterShapes(int numLayers, int ic, bool pairwise);
+  static std::vector<std::string> checkParameterFile(const std::string& npzPath, int numLayers, int ic,
+                                                     bool pairwise);
   void setVerbose(bool val) { m_verbose = val; }
   void setAllowGc(bool value) { m_allowGc = value; }
   // With d_pairwiseOut (a 1 x 2 x H x W float32 planar tensor, H x W the
@@ -338,8 +352,9 @@ class StereoNet {
   // COLORPAIR models (colour input only): performPrediction then writes wx, wy.
   virtual void initNet(float min_disp = 0, float max_disp = 0, float step = 0, int rect_corr = 0,
                        Tensor* d_unaryOut = nullptr, Tensor* d_pairwiseOut = nullptr);
-  // MI355X additions: tower arithmetic (BF16 MFMA by default) and the handle the
-  // network runs on.
+  // MI355X additions: tower arithmetic and the handle the network runs on.
+  // FP32 by default, as in the reference; Precision::BF16 (MFMA, FP32
+  // accumulation) is opt-in, about 3x faster with cost-volume errors near 1e-3.
   void setPrecision(Precision p);
   Handle& handle() { return *m_handle; }
   const std::vector<Operator*>& leftOps() const { return m_leftOps; }
@@ -370,7 +385,7 @@ class StereoNet {
   int m_pwCrop = 0;                 // pixels cropped from each side of the pairwise net's input
   int m_numLayers, m_in, m_ic, m_ih, m_iw;
   bool m_verbose = false, m_initialized = false, m_allowGc = false;
-  Precision m_precision = Precision::BF16;
+  Precision m_precision = Precision::FP32;
 };
 
 class ColorStereoNet : public StereoNet {
@@ -457,7 +472,7 @@ class Stereo {
   int num_cnn_layers_ = 3;
   std::string cnn_parameter_file_base_name_ = "params";
   MatchingFunction matching_function_ = COLORCNN;
-  Precision precision_ = Precision::BF16;
+  Precision precision_ = Precision::FP32;
   double lower_bound_ = 0.0;
   Tensor *img1_ = nullptr, *img2_ = nullptr;              // gray, H x W
   Tensor *img1_color_ = nullptr, *img2_color_ = nullptr;  // RGBA, H x W x 4

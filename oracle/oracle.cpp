@@ -357,6 +357,27 @@ extern "C" double or_energy(const double* C, const double* wx, const double* wy,
   return e;
 }
 
+// The energy over the labelled part of the grid: pixels whose mask label lies
+// outside [0, D) drop out with their unary and every edge they end.  The
+// structured-SVM energies of training with sparse ground truth (the reference's
+// training code is not in the repository; this is the definition the GPU's
+// masked energy and subgradient follow).
+extern "C" double or_energy_masked(const double* C, const double* wx, const double* wy, int H, int W, int D,
+                                   const int* labels, const int* mask, double L1, double L2, int delta) {
+  auto in = [&](long q) { return mask[q] >= 0 && mask[q] < D; };
+  double e = 0.0;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const long p = (long)y * W + x;
+      if (!in(p)) continue;
+      const int l = labels[p];
+      e += C[p * D + l];
+      if (x + 1 < W && in(p + 1)) e += wx[p] * or_rho(l - labels[p + 1], L1, L2, delta);
+      if (y + 1 < H && in(p + W)) e += wy[p] * or_rho(l - labels[p + W], L1, L2, delta);
+    }
+  return e;
+}
+
 namespace {
 
 // out(s) = min_t U(t) + w rho(s - t), exact for 0 <= L1 <= L2 and w >= 0:
@@ -603,9 +624,13 @@ extern "C" void or_ssvm_grad(const int* xhat, const int* xstar, const double* wx
   (void)L1;
   (void)L2;
   const long HW = (long)H * W;
+  // Pixels whose x* lies outside [0, D) are unlabelled: both energies leave
+  // out their unary and their edges (or_energy_masked with mask = x*).
+  auto lab = [&](long q) { return xstar[q] >= 0 && xstar[q] < D; };
   if (dC) {
     std::memset(dC, 0, sizeof(double) * HW * D);
     for (long p = 0; p < HW; ++p) {
+      if (!lab(p)) continue;
       dC[p * D + xstar[p]] += 1.0;
       dC[p * D + xhat[p]] -= 1.0;
     }
@@ -620,12 +645,12 @@ extern "C" void or_ssvm_grad(const int* xhat, const int* xstar, const double* wx
     for (int x = 0; x < W; ++x) {
       const long p = (long)y * W + x;
       double gx = 0.0, gy = 0.0;
-      if (x + 1 < W) {
+      if (x + 1 < W && lab(p) && lab(p + 1)) {
         d1 += wx[p] * (level(xstar[p], xstar[p + 1], 1) - level(xhat[p], xhat[p + 1], 1));
         d2 += wx[p] * (level(xstar[p], xstar[p + 1], 2) - level(xhat[p], xhat[p + 1], 2));
         gx = or_rho(xstar[p] - xstar[p + 1], L1, L2, delta) - or_rho(xhat[p] - xhat[p + 1], L1, L2, delta);
       }
-      if (y + 1 < H) {
+      if (y + 1 < H && lab(p) && lab(p + W)) {
         d1 += wy[p] * (level(xstar[p], xstar[p + W], 1) - level(xhat[p], xhat[p + W], 1));
         d2 += wy[p] * (level(xstar[p], xstar[p + W], 2) - level(xhat[p], xhat[p + W], 2));
         gy = or_rho(xstar[p] - xstar[p + W], L1, L2, delta) - or_rho(xhat[p] - xhat[p + W], L1, L2, delta);

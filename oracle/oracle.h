@@ -92,6 +92,11 @@ double or_rho(int d, double L1, double L2, int delta);
 double or_energy(const double* C, const double* wx, const double* wy, int H, int W, int D,
                  const int* labels, double L1, double L2, int delta);
 
+/* or_energy restricted to the pixels whose mask label lies in [0, D): the others
+ * drop out with their unary and every edge they end (training with sparse ground truth). */
+double or_energy_masked(const double* C, const double* wx, const double* wy, int H, int W, int D,
+                        const int* labels, const int* mask, double L1, double L2, int delta);
+
 /* Exact minimum of one chain (Viterbi), theta n x D, w n-1 edge weights.
  * Returns E*, writes an optimal labeling (lowest label on ties, backtracked). */
 double or_chain_viterbi(const double* theta, const double* w, int n, int D,
@@ -117,7 +122,8 @@ double or_slackprop(const double* C, const double* wx, const double* wy, int H, 
 
 /* Structured-SVM subgradient of  E(x*) - E(xhat)  w.r.t. unaries, L1, L2 and edge weights.
  * dC (H x W x D) gets +1 at x*, -1 at xhat; dL[0], dL[1] get dE/dL1, dE/dL2 differences;
- * dwx, dwy (H x W, optional) get rho differences. */
+ * dwx, dwy (H x W, optional) get rho differences.  Pixels with x* outside [0, D) are
+ * unlabelled and drop out of both energies (or_energy_masked with mask = x*). */
 void or_ssvm_grad(const int* xhat, const int* xstar, const double* wx, const double* wy,
                   int H, int W, int D, double L1, double L2, int delta,
                   double* dC, double* dL, double* dwx, double* dwy);

@@ -50,6 +50,7 @@ def lib():
             "or_lr_check": (None, [_D, _D, c_int, c_int, c_double, _D]),
             "or_rho": (c_double, [c_int, c_double, c_double, c_int]),
             "or_energy": (c_double, [_D, _D, _D, c_int, c_int, c_int, _I, c_double, c_double, c_int]),
+            "or_energy_masked": (c_double, [_D, _D, _D, c_int, c_int, c_int, _I, _I, c_double, c_double, c_int]),
             "or_chain_viterbi": (c_double, [_D, _D, c_int, c_int, c_double, c_double, c_int, _I]),
             "or_chain_minorant": (c_double, [_D, _D, c_int, c_int, c_double, c_double, c_int, _D, _D]),
             "or_slackprop": (c_double, [_D, _D, _D, c_int, c_int, c_int, c_double, c_double, c_int,
@@ -225,6 +226,15 @@ def energy(C, wx, wy, labels, L1, L2, delta):
     H, W, D = C.shape
     lab = _i(labels)
     return lib().or_energy(_pd(C), _pd(wx), _pd(wy), H, W, D, _pi(lab), L1, L2, int(delta))
+
+
+def energy_masked(C, wx, wy, labels, mask, L1, L2, delta):
+    """Energy over the pixels whose mask label lies in [0, D) (their unaries and
+    the edges between two of them)."""
+    C, wx, wy = _d(C), _d(wx), _d(wy)
+    H, W, D = C.shape
+    lab, m = _i(labels), _i(mask)
+    return lib().or_energy_masked(_pd(C), _pd(wx), _pd(wy), H, W, D, _pi(lab), _pi(m), L1, L2, int(delta))
 
 
 def chain_viterbi(theta, w, L1, L2, delta):

@@ -1,0 +1,167 @@
+"""C-ABI robustness and the statics added beside the reference's interface:
+
+* ccs_slackprop_forward_volume_out (SlackProp::forwardVolumeOut,
+  stereo/operators/slackprop.h:46-47, slackprop.cpp:142-173) against the
+  oracle's final row unaries and bound;
+* NaN costs reported as CCS_ERR_NAN by the synchronising statics and by
+  ccs_slackprop_status after the stream-ordered ccs_slackprop;
+* scratch growth refused while the handle's stream is being captured, and a
+  capture after ccs_reserve replaying to the eager result;
+* the structured-SVM hinge with unlabelled pixels (targets -1) against the
+  oracle's masked energies and subgradient;
+* the convolution adjoint with more than 1024 output channels (bias gradient).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from cnn_crf_stereo_amd import autograd as AG
+from cnn_crf_stereo_amd import capi, ops
+from cnn_crf_stereo_amd.capi import call, ptr
+
+pytestmark = pytest.mark.gpu
+
+
+def crf_case(seed, H, W, D, sharp=5.0):
+    rng = np.random.default_rng(seed)
+    C = O.neg_softmax(rng.normal(size=(H, W, D)) * sharp).astype(np.float32)
+    wx = rng.uniform(0.2, 1.0, size=(H, W)).astype(np.float32)
+    wy = rng.uniform(0.2, 1.0, size=(H, W)).astype(np.float32)
+    return C, wx, wy
+
+
+def dev(a):
+    return torch.as_tensor(a, device="cuda")
+
+
+@pytest.mark.parametrize("H,W,D,iters,delta", [(6, 40, 16, 3, 2), (5, 33, 128, 2, 3), (9, 14, 64, 1, 2)])
+def test_forward_volume_out_matches_oracle(handle, H, W, D, iters, delta):
+    C, wx, wy = crf_case(300 + D, H, W, D)
+    out, lb = ops.slackprop_forward_volume_out(handle, dev(C), dev(wx), dev(wy), 0.5, 2.0, iters, delta)
+    r = O.slackprop(C, wx, wy, 0.5, 2.0, delta, iters, want_A=True)
+    assert abs(lb - r["lb"]) <= 1e-5 * max(1.0, abs(r["lb"]))
+    A = out.cpu().numpy()
+    np.testing.assert_allclose(A, r["A"], atol=2e-5 * max(1.0, np.abs(r["A"]).max()), rtol=0)
+    # the same volume as ccs_slackprop's unaries_out
+    g = ops.slackprop(handle, dev(C), dev(wx), dev(wy), 0.5, 2.0, delta, iters, want_unaries=True)
+    assert torch.equal(g["unaries"], out)
+
+
+def test_nan_cost_is_reported(handle):
+    C, wx, wy = crf_case(310, 7, 30, 32)
+    Cn = C.copy()
+    Cn[4, 11, 5] = np.nan
+    with pytest.raises(capi.CCSError) as e:
+        ops.slackprop_forward_static(handle, dev(Cn), dev(wx), dev(wy), 0.5, 2.0, 2)
+    assert e.value.status == capi.ERR_NAN and "y = 4" in str(e.value)
+    with pytest.raises(capi.CCSError) as e:
+        ops.slackprop_forward_volume_out(handle, dev(Cn), dev(wx), dev(wy), 0.5, 2.0, 2, 2)
+    assert e.value.status == capi.ERR_NAN
+    g = ops.slackprop(handle, dev(Cn), dev(wx), dev(wy), 0.5, 2.0, 2, 2)  # stream-ordered: no status yet
+    with pytest.raises(capi.CCSError) as e:
+        ops.slackprop_status(handle)
+    assert e.value.status == capi.ERR_NAN
+    lab = g["labels"].cpu().numpy()
+    assert lab.min() >= 0 and lab.max() < 32  # unspecified labels, but in range
+    # a clean volume clears the status; the static returns the bound again
+    ops.slackprop(handle, dev(C), dev(wx), dev(wy), 0.5, 2.0, 2, 2)
+    ops.slackprop_status(handle)
+    out, lb = ops.slackprop_forward_static(handle, dev(C), dev(wx), dev(wy), 0.5, 2.0, 2)
+    assert abs(lb - O.slackprop(C, wx, wy, 0.5, 2.0, 2, 2)["lb"]) <= 1e-4
+    # +inf costs are numbers to the CRF, not NaN
+    Ci = C.copy()
+    Ci[2, 3, :7] = np.inf
+    ops.slackprop(handle, dev(Ci), dev(wx), dev(wy), 0.5, 2.0, 2, 2)
+    ops.slackprop_status(handle)
+
+
+def test_scratch_growth_refused_under_capture_then_reserve():
+    H, W, D, iters = 24, 70, 32, 2
+    C, wx, wy = crf_case(320, H, W, D)
+    Cd, wxd, wyd = dev(C), dev(wx), dev(wy)
+    stream = torch.cuda.Stream()
+    h = capi.Handle(stream)
+    labels = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    sums = torch.zeros(4 * iters + 1, dtype=torch.float64, device="cuda")
+    lb = torch.zeros(1, dtype=torch.float64, device="cuda")
+    args = (h.h, ptr(Cd), ptr(wxd), ptr(wyd), H, W, D, 0.5, 2.0, 2, iters, ptr(labels), None, ptr(sums), ptr(lb))
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(capi.CCSError) as e:
+        with torch.cuda.graph(g, stream=stream):
+            call("ccs_slackprop", *args)  # a fresh handle's scratch would have to grow
+    assert e.value.status == capi.ERR_ARG and "ccs_reserve" in str(e.value)
+    torch.cuda.synchronize()
+    call("ccs_reserve", h.h, H, W, D, 100, iters)
+    with torch.cuda.stream(stream):
+        call("ccs_slackprop", *args)
+    torch.cuda.synchronize()
+    want = (labels.clone(), sums.clone(), lb.clone())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        call("ccs_slackprop", *args)
+    for _ in range(3):
+        labels.zero_()
+        sums.zero_()
+        lb.zero_()
+        with torch.cuda.stream(stream):
+            g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(labels, want[0]) and torch.equal(sums, want[1]) and torch.equal(lb, want[2])
+    h.close()
+
+
+def test_masked_hinge_with_unlabelled_pixels(handle):
+    rng = np.random.default_rng(330)
+    H, W, D = 12, 20, 16
+    C = O.neg_softmax(rng.normal(size=(H, W, D)) * 4.0).astype(np.float32)
+    wx = rng.uniform(0.2, 1, (H, W)).astype(np.float32)
+    wy = rng.uniform(0.2, 1, (H, W)).astype(np.float32)
+    t = rng.integers(0, D, (H, W)).astype(np.int32)
+    t[rng.uniform(size=(H, W)) < 0.35] = -1
+    t[:, :4] = -1  # no ground truth at the left border
+    params = torch.tensor([0.5, 2.0], device="cuda", requires_grad=True)
+    cost = dev(C).requires_grad_()
+    hinge = AG.SsvmHinge.apply(cost, dev(wx), dev(wy), params, dev(t), 4, 2, 1.0, handle)
+    hinge.backward()
+    # the most violating labelling of the augmented unaries (unlabelled pixels keep C)
+    aug = (C - 1.0 * ((np.arange(D)[None, None, :] != t[..., None]) & (t[..., None] >= 0))).astype(np.float32)
+    xhat = ops.slackprop(handle, dev(aug), dev(wx), dev(wy), 0.5, 2.0, 2, 4)["labels"].cpu().numpy()
+    r = O.slackprop(aug, wx, wy, 0.5, 2.0, 2, 4)
+    assert np.all(r["gap"][xhat != r["labels"]] < 1e-3)
+    tc = np.clip(t, 0, D - 1)
+    e_star = O.energy_masked(C, wx, wy, tc, t, 0.5, 2.0, 2)
+    e_hat = O.energy_masked(aug, wx, wy, xhat, t, 0.5, 2.0, 2)
+    assert abs(float(hinge.detach()) - (e_star - e_hat)) <= 1e-4 * max(1.0, abs(e_star))
+    dC, dL, _, _ = O.ssvm_grad(xhat, t, wx, wy, D, 0.5, 2.0, 2)
+    np.testing.assert_allclose(params.grad.cpu().numpy(), dL, rtol=1e-5, atol=1e-4)
+    np.testing.assert_array_equal(cost.grad.cpu().numpy(), dC)
+    assert not cost.grad[torch.as_tensor(t < 0, device="cuda")].any()  # nothing at unlabelled pixels
+
+
+def test_conv_backward_many_output_channels(handle):
+    rng = np.random.default_rng(340)
+    H, W, C, K = 7, 9, 3, 1100
+    x = rng.normal(size=(C, H, W)) * 0.5
+    w = rng.normal(size=(K, C, 3, 3)) * 0.2
+    b = rng.normal(size=K) * 0.1
+    Ho, Wo = H - 2, W - 2
+    dy = rng.normal(size=(K, Ho, Wo))
+    cs, ks = capi.feature_stride(C), capi.feature_stride(K)
+    xh = np.zeros((H, W, cs), np.float32)
+    xh[..., :C] = x.transpose(1, 2, 0)
+    dyh = np.zeros((Ho, Wo, ks), np.float32)
+    dyh[..., :K] = dy.transpose(1, 2, 0)
+    wf, bf = w.astype(np.float32), b.astype(np.float32)
+    dx = torch.zeros((H, W, cs), device="cuda")
+    dw = torch.empty((K, C, 3, 3), device="cuda")
+    db = torch.empty(K, device="cuda")
+    xd, wd, bd, dyd = dev(xh), dev(wf), dev(bf), dev(dyh)
+    call("ccs_conv2d_backward", handle.h, ptr(xd), H, W, C, ptr(wd), ptr(bd), K, 3, 3, 1, ptr(dyd), ptr(dx), ptr(dw),
+         ptr(db))
+    rdx, rdw, rdb = O.conv2d_bwd(xh[..., :C].transpose(2, 0, 1).astype(np.float64), wf.astype(np.float64),
+                                 bf.astype(np.float64), dy, True)
+    np.testing.assert_allclose(db.cpu().numpy(), rdb, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(dw.cpu().numpy(), rdw, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(dx.cpu().numpy()[..., :C].transpose(2, 0, 1), rdx, rtol=1e-4, atol=1e-3)

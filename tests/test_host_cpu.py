@@ -89,6 +89,7 @@ def test_option_defaults_and_cli_over_file(tmp_path):
     d = options("a.png", "b.png")
     assert (d["disp-min"], d["disp-max"], d["matching"], d["inference"], d["lambda"], d["L1"], d["L2"]) == \
         ("1", "128", "CENSUS", "CRF", "0.4", "1", "12")
+    assert d["precision"] == "fp32"  # the reference's arithmetic unless bf16 is asked for
     cfg = tmp_path / "m.cfg"
     cfg.write_text("matching = COLORCNN\nnum-layers = 7\nlambda = 0.3\n# comment\nL1 = 0.9986288 # learned\n"
                    "L2 = 11.95378876\ncrf-iterations = 5\n")

@@ -421,3 +421,63 @@ def test_ssvm_grad_closed_form():
     e[1, 2] = 1.0
     np.testing.assert_allclose(dwx[1, 2], diff(L1, L2, wx + e, wy) - base, atol=1e-12)
     np.testing.assert_allclose(dwy[1, 2], diff(L1, L2, wx, wy + e) - base, atol=1e-12)
+
+
+def test_masked_energy_equals_energy_with_dropped_terms():
+    """or_energy_masked over the labelled pixels equals or_energy of the same
+    labelling once the unlabelled pixels' unaries and every edge weight that
+    touches them are set to zero; with every pixel labelled it is or_energy."""
+    rng = np.random.default_rng(12)
+    H, W, D = 5, 6, 7
+    C = rng.normal(size=(H, W, D))
+    wx, wy = rng.uniform(0.1, 1, (H, W)), rng.uniform(0.1, 1, (H, W))
+    lab = rng.integers(0, D, (H, W))
+    mask = rng.integers(0, D, (H, W))
+    np.testing.assert_allclose(O.energy_masked(C, wx, wy, lab, mask, 0.5, 2.0, 2), O.energy(C, wx, wy, lab, 0.5, 2.0, 2),
+                               rtol=1e-14)
+    mask[rng.uniform(size=(H, W)) < 0.3] = -1
+    mask[0, 0], mask[2, 3] = -1, D  # both sides of the range
+    out = mask < 0
+    out |= mask >= D
+    C0, wx0, wy0 = C.copy(), wx.copy(), wy.copy()
+    C0[out] = 0.0
+    wx0[out] = 0.0
+    wx0[:, :-1][out[:, 1:]] = 0.0  # edge (x, y)-(x+1, y) ends at an unlabelled pixel
+    wy0[out] = 0.0
+    wy0[:-1, :][out[1:, :]] = 0.0
+    np.testing.assert_allclose(O.energy_masked(C, wx, wy, lab, mask, 0.5, 2.0, 2),
+                               O.energy(C0, wx0, wy0, lab, 0.5, 2.0, 2), rtol=1e-12)
+
+
+def test_ssvm_grad_with_unlabelled_pixels():
+    """With ground truth missing at some pixels (x* outside [0, D)) the
+    subgradient is that of the masked margin E_m(x*) - E_m(xhat): 0 on the
+    unlabelled pixels' unaries, and exact finite differences of the masked
+    energies in L1, L2 and the edge weights."""
+    rng = np.random.default_rng(13)
+    H, W, D = 5, 6, 6
+    xh = rng.integers(0, D, (H, W))
+    xs = rng.integers(0, D, (H, W))
+    xs[rng.uniform(size=(H, W)) < 0.3] = -1
+    xs[1, 2] = -1
+    wx, wy = rng.uniform(0, 1, (H, W)), rng.uniform(0, 1, (H, W))
+    L1, L2 = 0.5, 2.0
+    dC, dL, dwx, dwy = O.ssvm_grad(xh, xs, wx, wy, D, L1, L2, 2)
+    lab = xs >= 0
+    want = np.where(lab[..., None], np.eye(D)[np.clip(xs, 0, D - 1)] - np.eye(D)[xh], 0.0)
+    np.testing.assert_array_equal(dC, want)
+    zero = np.zeros((H, W, D))
+    xsc = np.clip(xs, 0, D - 1)
+
+    def diff(L1_, L2_, wx_, wy_):
+        return (O.energy_masked(zero, wx_, wy_, xsc, xs, L1_, L2_, 2)
+                - O.energy_masked(zero, wx_, wy_, xh, xs, L1_, L2_, 2))
+
+    base = diff(L1, L2, wx, wy)
+    np.testing.assert_allclose(dL[0], diff(L1 + 1, L2, wx, wy) - base, atol=1e-12)
+    np.testing.assert_allclose(dL[1], diff(L1, L2 + 1, wx, wy) - base, atol=1e-12)
+    for (y, x) in [(1, 1), (1, 2), (3, 4)]:
+        e = np.zeros((H, W))
+        e[y, x] = 1.0
+        np.testing.assert_allclose(dwx[y, x], diff(L1, L2, wx + e, wy) - base, atol=1e-12)
+        np.testing.assert_allclose(dwy[y, x], diff(L1, L2, wx, wy + e) - base, atol=1e-12)
