@@ -337,9 +337,16 @@ def stage_errors(ops, h, fr, out, cb):
     }
 
 
-def train_bench(pkg, h, workload, layers_n, arch, batch, steps, warmup, iters, seed=11):
+def train_bench(pkg, h, workload, layers_n, arch, batch, steps, warmup, iters, precision="fp32", checkpoint=True,
+                grad_check=True, seed=11):
     """Training steps (forward, CRF hinge, backward through the hand-written
-    adjoints, SGD) on `batch` seeded frames of `workload`; FP32 master weights."""
+    adjoints, SGD) on `batch` seeded frames of `workload` with FP32 master
+    weights, the towers and the correlation in `precision`, checkpointed towers
+    (only the padded images kept; layer outputs recomputed in the backward) or
+    store-everything layers.  Reports ms/step, the peak of torch's device
+    allocations over the timed steps, and the relative error (L2 over all
+    tower and CRF parameters) of the first step's gradient against the FP32
+    store-everything backward at the same weights."""
     import numpy as np
     import torch
     from cnn_crf_stereo_amd import autograd as AG
@@ -348,24 +355,42 @@ def train_bench(pkg, h, workload, layers_n, arch, batch, steps, warmup, iters, s
     spec = dict(synth.WORKLOADS[workload])
     D = spec["D"]
     layers, _ = synth.architecture(layers_n, spec["C"], arch)
-    model = AG.StereoModel(synth.glorot_weights(layers, seed=seed))
+    params = synth.glorot_weights(layers, seed=seed)
     frames = []
     for i in range(batch):
         d = synth.stereo_pair(spec, seed=2000 + i)
         tgt = np.clip(np.rint(d["disp"]), 0, D - 1).astype(np.int32)
         frames.append((torch.as_tensor(d["left"], device="cuda"), torch.as_tensor(d["right"], device="cuda"),
                        torch.as_tensor(tgt, device="cuda")))
+    out = {"workload": workload, "batch": batch, "layers": "%d x %s, 100 channels" % (layers_n, arch),
+           "crf_iterations": iters, "dtype": precision,
+           "towers": "checkpointed (padded image kept, layer outputs recomputed)" if checkpoint else
+                     "store-everything"}
+    if grad_check:
+        ref = AG.StereoModel(params, precision="fp32", checkpoint=False)
+        _, g_ref = AG.gradients(ref, h, frames, D, iters=iters)
+        del ref
+        model = AG.StereoModel(params, precision=precision, checkpoint=checkpoint)
+        _, g = AG.gradients(model, h, frames, D, iters=iters)
+        num = sum(float(((a - b).double() ** 2).sum()) for a, b in zip(g, g_ref))
+        den = sum(float((b.double() ** 2).sum()) for b in g_ref)
+        per = [float((a - b).double().norm() / max(float(b.double().norm()), 1e-300)) for a, b in zip(g, g_ref)]
+        out["grad_rel_err_vs_fp32_store_everything"] = float(np.sqrt(num / max(den, 1e-300)))
+        out["grad_rel_err_max_per_tensor"] = max(per)
+        del g, g_ref
+    model = AG.StereoModel(params, precision=precision, checkpoint=checkpoint)
     for _ in range(warmup):
         AG.training_step(model, h, frames, D, lr=1e-4, iters=iters)
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     losses = [AG.training_step(model, h, frames, D, lr=1e-4, iters=iters) for _ in range(steps)]
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
-    free, total = torch.cuda.mem_get_info()
-    return {"workload": workload, "batch": batch, "layers": "%d x %s, 100 channels" % (layers_n, arch),
-            "crf_iterations": iters, "ms_per_step": round(ms, 2), "loss": [round(v, 5) for v in losses],
-            "device_memory_in_use_gb": round((total - free) / 1e9, 2), "dtype": "fp32"}
+    out.update({"ms_per_step": round(ms, 2), "loss": [round(v, 5) for v in losses],
+                "peak_torch_allocated_gb": round(torch.cuda.max_memory_allocated() / 1e9, 3)})
+    return out
 
 
 def main():
@@ -536,7 +561,12 @@ def main():
     if not args.no_train:
         result_extra["train_step"] = [
             train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=3, warmup=1, iters=8),
-            train_bench(pkg, h, "rds-1488x992-d256", 7, "all3x3", batch=1, steps=2, warmup=1, iters=10),
+            train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=3, warmup=1, iters=8,
+                        checkpoint=False, grad_check=False),
+            train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=3, warmup=1, iters=8,
+                        precision="bf16"),
+            train_bench(pkg, h, "rds-1488x992-d256", 7, "all3x3", batch=1, steps=2, warmup=1, iters=10,
+                        precision="bf16"),
         ]
 
     line = {

@@ -4,9 +4,15 @@ Every forward and backward computation runs in libccs.so; torch supplies device
 memory, the stream and gradient accumulation across the two towers (the
 weights are shared) and across the frames of a batch.
 
-  ConvTanh      ccs_conv2d_forward / ccs_conv2d_backward (the backward
-                recomputes the pre-activation from the layer input instead of
-                storing it)
+  Tower         one feature tower, checkpointed: ccs_conv2d_forward per layer
+                (float32 or bfloat16 maps, float32 master weights), saving only
+                the padded image; the backward recomputes the layer outputs and
+                runs ccs_conv2d_backward_stored from the last layer down,
+                releasing each output after its adjoint
+  ConvTanh      one layer, store-everything: ccs_conv2d_forward /
+                ccs_conv2d_backward (float32; keeps the layer input, recomputes
+                the output in the backward) -- the plain form the checkpointed
+                tower is compared against
   Correlation   ccs_correlation_volume (raw scores) / ccs_correlation_volume_backward
                 (each pixel gathers its disparity band: no atomics)
   NegSoftmax    ccs_softmax_forward(negate) / ccs_neg_softmax_backward
@@ -21,11 +27,70 @@ elsewhere); these adjoints have no reference counterpart.
 import torch
 
 from . import ops
-from .capi import EPILOGUE_RAW, FP32, LAYOUT_NCHW, LAYOUT_NHWC, call, feature_stride, ptr
+from .capi import BF16, EPILOGUE_RAW, FP32, LAYOUT_NCHW, LAYOUT_NHWC, call, feature_stride, ptr
+
+_DTYPE = {"fp32": torch.float32, "bf16": torch.bfloat16}
+_CODE = {"fp32": FP32, "bf16": BF16}
 
 
-def _zeros_hwc(H, W, C):
-    return torch.zeros((H, W, feature_stride(C)), dtype=torch.float32, device="cuda")
+def _zeros_hwc(H, W, C, dtype=torch.float32):
+    return torch.zeros((H, W, feature_stride(C)), dtype=dtype, device="cuda")
+
+
+def _tower_forward(h, img, ws, bs, precision, keep):
+    """Layer outputs of one tower from the padded planar image: all of them
+    (keep) or the last one only (each earlier map is released when the next
+    layer has been queued)."""
+    x, outs = img, []
+    for l, (w, b) in enumerate(zip(ws, bs)):
+        K, C, KH, KW = w.shape
+        H, W = x.shape[1:] if l == 0 else x.shape[:2]
+        y = torch.empty((H - KH + 1, W - KW + 1, feature_stride(K)), dtype=_DTYPE[precision], device="cuda")
+        call("ccs_conv2d_forward", h.h, ptr(x), LAYOUT_NCHW if l == 0 else LAYOUT_NHWC, H, W, C, ptr(w), ptr(b), K,
+             KH, KW, 1, _CODE[precision], ptr(y))
+        if keep:
+            outs.append(y)
+        x = y
+    return outs if keep else [x]
+
+
+class Tower(torch.autograd.Function):
+    """Feature tower y = layer_L(... layer_1(img)), fused conv + bias + tanh per
+    layer, in float32 or bfloat16 (MFMA, float32 accumulation) from float32
+    master weights.  Checkpointed: only the padded image (C x Hp x Wp) is
+    saved.  The backward recomputes the layer outputs in the forward's
+    precision (the same kernels, so the same values), then runs the adjoints
+    from the last layer down; tanh' comes from the kept outputs, and each
+    output is released once its layer's adjoint is queued.  Peak memory is one
+    tower's maps during the backward instead of every layer of both towers
+    from the forward to the backward."""
+
+    @staticmethod
+    def forward(ctx, img, precision, h, *wb):
+        ws, bs = wb[0::2], wb[1::2]
+        y = _tower_forward(h, img, ws, bs, precision, keep=False)[-1]
+        ctx.save_for_backward(img, *wb)
+        ctx.precision, ctx.h = precision, h
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        img, *wb = ctx.saved_tensors
+        ws, bs = wb[0::2], wb[1::2]
+        h, precision = ctx.h, ctx.precision
+        acts = _tower_forward(h, img, ws, bs, precision, keep=True)
+        C0, Hp, Wp = img.shape
+        x0 = _zeros_hwc(Hp, Wp, C0)  # the first layer's adjoint reads the image channel-last
+        call("ccs_nchw_to_nhwc", h.h, ptr(img.contiguous()), C0, feature_stride(C0), Hp, Wp, ptr(x0))
+        grads = [None] * len(wb)
+        g = dy.to(_DTYPE[precision]).contiguous()
+        for l in reversed(range(len(ws))):
+            x = acts[l - 1] if l > 0 else x0
+            dx, dw, db = ops.conv2d_backward_stored(h, x, acts[l], ws[l], g, True, want_dx=l > 0)
+            grads[2 * l], grads[2 * l + 1] = dw, db
+            acts[l] = None  # this layer's output is no longer needed
+            g = dx
+        return (None, None, None, *grads)
 
 
 class ConvTanh(torch.autograd.Function):
@@ -73,14 +138,12 @@ class ConvTanh(torch.autograd.Function):
 
 
 class Correlation(torch.autograd.Function):
-    """S[h][w][k] = <f0[h][w], f1[h][w - (disp_min + k disp_step)]> (raw scores)."""
+    """S[h][w][k] = <f0[h][w], f1[h][w - (disp_min + k disp_step)]> (raw scores);
+    float32 or bfloat16 maps (MFMA), float32 scores."""
 
     @staticmethod
     def forward(ctx, f0, f1, C, disp_min, disp_step, D, h):
-        H, W = f0.shape[:2]
-        S = torch.empty((H, W, D), dtype=torch.float32, device="cuda")
-        call("ccs_correlation_volume", h.h, ptr(f0), ptr(f1), FP32, H, W, C, int(disp_min), int(disp_step), D,
-             EPILOGUE_RAW, ptr(S))
+        S = ops.correlation_volume(h, f0, f1, C, disp_min, disp_step, D, epilogue="raw")
         ctx.save_for_backward(f0, f1)
         ctx.args = (C, disp_min, disp_step, D, h)
         return S
@@ -89,11 +152,7 @@ class Correlation(torch.autograd.Function):
     def backward(ctx, G):
         f0, f1 = ctx.saved_tensors
         C, disp_min, disp_step, D, h = ctx.args
-        H, W = f0.shape[:2]
-        df0 = torch.zeros_like(f0)
-        df1 = torch.zeros_like(f1)
-        call("ccs_correlation_volume_backward", h.h, ptr(f0), ptr(f1), H, W, C, int(disp_min), int(disp_step), D,
-             ptr(G.contiguous()), ptr(df0), ptr(df1))
+        df0, df1 = ops.correlation_volume_backward(h, f0, f1, C, disp_min, disp_step, D, G.contiguous())
         return df0, df1, None, None, None, None, None
 
 
@@ -166,19 +225,28 @@ class SsvmHinge(torch.autograd.Function):
 
 
 class StereoModel:
-    """Feature tower weights (shared by both eyes) and the learned L1, L2, as
-    leaf tensors with requires_grad."""
+    """Feature tower weights (shared by both eyes, float32 masters) and the
+    learned L1, L2, as leaf tensors with requires_grad.
 
-    def __init__(self, params, L1=0.5, L2=2.0):
+    precision "fp32" or "bf16" is the towers' and the correlation's arithmetic;
+    checkpoint=True runs each tower as one checkpointed Tower, False as
+    store-everything ConvTanh layers (float32 only)."""
+
+    def __init__(self, params, L1=0.5, L2=2.0, precision="fp32", checkpoint=True):
+        if precision not in _DTYPE or (precision == "bf16" and not checkpoint):
+            raise ValueError("precision must be fp32, or bf16 with checkpoint=True")
         self.weights = [(torch.as_tensor(w, device="cuda").clone().requires_grad_(),
                          torch.as_tensor(b, device="cuda").clone().requires_grad_()) for (w, b) in params]
         self.crf = torch.tensor([L1, L2], dtype=torch.float32, device="cuda", requires_grad=True)
         self.pad = sum(w.shape[2] - 1 for (w, _) in self.weights) // 2
+        self.precision, self.checkpoint = precision, checkpoint
 
     def parameters(self):
         return [t for wb in self.weights for t in wb] + [self.crf]
 
     def tower(self, h, img_padded):
+        if self.checkpoint:
+            return Tower.apply(img_padded, self.precision, h, *[t for wb in self.weights for t in wb])
         x = img_padded
         for i, (w, b) in enumerate(self.weights):
             x = ConvTanh.apply(x, w, b, h, i == 0)
@@ -205,6 +273,22 @@ def frame_loss(model, h, left, right, target, D, iters=4, delta=2, hinge_weight=
         wx, wy = ops.edge_weights(h, left[0], sigma, alpha, beta, lam)
         loss = loss + hinge_weight * SsvmHinge.apply(cost, wx, wy, model.crf, target, iters, delta, 1.0, h) / (H * W)
     return loss
+
+
+def gradients(model, h, batch, D, **kw):
+    """Mean loss over a batch of (left, right, target) frames and the gradients
+    of every parameter (float32 copies), without updating the model."""
+    for p in model.parameters():
+        p.grad = None
+    total = 0.0
+    for (left, right, target) in batch:
+        loss = frame_loss(model, h, left, right, target, D, **kw) / len(batch)
+        loss.backward()
+        total += float(loss.detach())
+    grads = [p.grad.detach().clone() if p.grad is not None else torch.zeros_like(p) for p in model.parameters()]
+    for p in model.parameters():
+        p.grad = None
+    return total, grads
 
 
 def training_step(model, h, batch, D, lr=1e-3, **kw):

@@ -36,10 +36,11 @@ SIGNATURES = {
     "ccs_conv_pack_weights": (_I, [_P, _P, _I, _I, _I, _I, _I, _P]),
     "ccs_conv_forward_packed": (_I, [_P, _P, _I, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P]),
     "ccs_conv2d_backward": (_I, [_P, _P, _I, _I, _I, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "ccs_conv2d_backward_stored": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, _P, _I, _I, _I, _I, _P, _P, _P, _P]),
     "ccs_correlation_forward": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I]),
     "ccs_correlation_backward": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I]),
     "ccs_correlation_volume": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P]),
-    "ccs_correlation_volume_backward": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _P]),
+    "ccs_correlation_volume_backward": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]),
     "ccs_softmax_xent": (_I, [_P, _P, _I, _I, _P, _P, _P]),
     "ccs_edge_weights": (_I, [_P, _P, _I, _I, _F, _F, _F, _F, _P, _P]),
     "ccs_slackprop_forward": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _F, _I, ctypes.POINTER(_F)]),

@@ -60,6 +60,7 @@ struct ccs_handle_s {
   ccs::Scratch crf_sums;   // per-half-step sums of ccs_slackprop
   ccs::Scratch misc;       // small temporaries
   ccs::Scratch misc2;
+  ccs::Scratch conv_y;     // recomputed layer output of ccs_conv2d_backward
   ccs::Scratch crf_nan;    // per row chain: NaN seen by the first row sweep of the last run
   int nan_chains = 0;      // rows flagged in crf_nan by that sweep (0: none ran)
 };

@@ -245,10 +245,12 @@ __global__ void k_corr_bwd_generic(float* __restrict__ d0, float* __restrict__ d
   d1[idx] = g1;
 }
 
-// Adjoint of the fast path (float32 channel-last maps): one thread per (h, w, c).
-__global__ void k_corr_volume_bwd(const float* __restrict__ f0, const float* __restrict__ f1, int H, int W, int C,
+// Adjoint of the fast path (channel-last maps, float32 or bfloat16, gradients
+// in the maps' type, float32 accumulation): one thread per (h, w, c).
+template <typename T>
+__global__ void k_corr_volume_bwd(const T* __restrict__ f0, const T* __restrict__ f1, int H, int W, int C,
                                   int Cs, int o0, int ostep, int D, const float* __restrict__ G,
-                                  float* __restrict__ df0, float* __restrict__ df1) {
+                                  T* __restrict__ df0, T* __restrict__ df1) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)H * W * Cs) return;
   const int c = (int)(idx % Cs);
@@ -260,13 +262,13 @@ __global__ void k_corr_volume_bwd(const float* __restrict__ f0, const float* __r
     for (int k = 0; k < D; ++k) {
       const int o = o0 + k * ostep;
       const int wp = w + o;
-      if (wp >= 0 && wp < W) g0 += f1[(row + wp) * Cs + c] * G[(row + w) * D + k];
+      if (wp >= 0 && wp < W) g0 += (float)f1[(row + wp) * Cs + c] * G[(row + w) * D + k];
       const int wm = w - o;
-      if (wm >= 0 && wm < W) g1 += f0[(row + wm) * Cs + c] * G[(row + wm) * D + k];
+      if (wm >= 0 && wm < W) g1 += (float)f0[(row + wm) * Cs + c] * G[(row + wm) * D + k];
     }
   }
-  df0[idx] = g0;
-  df1[idx] = g1;
+  df0[idx] = (T)g0;
+  df1[idx] = (T)g1;
 }
 
 // Cross entropy of softmax_D(S) at the target label; dS = softmax - onehot.
@@ -447,15 +449,22 @@ extern "C" int ccs_correlation_backward(ccs_handle_t h, float* out_grad0, float*
   return CCS_OK;
 }
 
-extern "C" int ccs_correlation_volume_backward(ccs_handle_t h, const float* f0, const float* f1, int H, int W, int C,
-                                               int disp_min, int disp_step, int D, const float* G, float* df0,
-                                               float* df1) {
+extern "C" int ccs_correlation_volume_backward(ccs_handle_t h, const void* f0, const void* f1, int precision, int H,
+                                               int W, int C, int disp_min, int disp_step, int D, const float* G,
+                                               void* df0, void* df1) {
   CCS_CHECK(h && f0 && f1 && G && df0 && df1, CCS_ERR_ARG, "ccs_correlation_volume_backward: NULL argument");
   CCS_CHECK(H >= 1 && W >= 1 && C >= 1 && D >= 1, CCS_ERR_ARG, "ccs_correlation_volume_backward: empty shape");
+  CCS_CHECK(precision == CCS_FP32 || precision == CCS_BF16, CCS_ERR_ARG, "ccs_correlation_volume_backward: precision");
   const int Cs = feature_stride(C);
   const long total = (long)H * W * Cs;
-  k_corr_volume_bwd<<<ceil_div(total, 256), 256, 0, h->stream>>>(f0, f1, H, W, C, Cs, -disp_min, -disp_step, D, G,
-                                                                df0, df1);
+  if (precision == CCS_BF16)
+    k_corr_volume_bwd<bf16><<<ceil_div(total, 256), 256, 0, h->stream>>>((const bf16*)f0, (const bf16*)f1, H, W, C,
+                                                                        Cs, -disp_min, -disp_step, D, G, (bf16*)df0,
+                                                                        (bf16*)df1);
+  else
+    k_corr_volume_bwd<float><<<ceil_div(total, 256), 256, 0, h->stream>>>((const float*)f0, (const float*)f1, H, W,
+                                                                         C, Cs, -disp_min, -disp_step, D, G,
+                                                                         (float*)df0, (float*)df1);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }

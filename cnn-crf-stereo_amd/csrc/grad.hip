@@ -1,33 +1,48 @@
-// Adjoint of the fused convolution + bias + tanh layer (float32 feature maps).
-// The reference has no convolution backward; this follows the forward
-// definition y = tanh(b + sum_{c,i,j} x[h+i][w+j][c] W[k][c][KH-1-i][KW-1-j]):
-//   dz = dy * (1 - y^2)                (y recomputed from x, not stored)
+// Adjoint of the fused convolution + bias + tanh layer.  The reference has no
+// convolution backward; this follows the forward definition
+// y = tanh(b + sum_{c,i,j} x[h+i][w+j][c] W[k][c][KH-1-i][KW-1-j]):
+//   dz = dy * (1 - y^2)                (y: the layer's output, kept or recomputed)
 //   db[k] = sum_p dz[p][k]
 //   dW[k][c][KH-1-i][KW-1-j] = sum_p dz[p][k] x[p + (i,j)][c]
 //   dx = true convolution of dz zero-padded by (KH-1, KW-1) with
 //        Wt[c][k][a][b] = W[k][c][KH-1-a][KW-1-b]
-// Reductions over pixels are split into blocks and summed deterministically.
+// dW runs on v_mfma_f32_16x16x4_f32 (k_dw_mfma: per tap, a K x C GEMM reduced
+// over pixels), dx on the forward MFMA kernels.  Maps are float32 or bfloat16
+// (the BF16 form keeps activations and activation gradients in bf16 and
+// accumulates in float32).  Reductions over pixels are split into blocks and
+// summed in a fixed order: results do not depend on scheduling.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace ccs {
 
-__global__ void k_dz(const float* __restrict__ y, const float* __restrict__ dy, long n, int act,
-                     float* __restrict__ dz) {
+template <typename T>
+__device__ __forceinline__ float ldf(const T* p, long i) {
+  return (float)p[i];
+}
+
+// dz = dy (1 - y^2) (act) or dy, for the K live channels; 0 in the pad channels.
+template <typename T>
+__global__ void k_dz(const T* __restrict__ y, const T* __restrict__ dy, long n, int Cs, int K, int act,
+                     T* __restrict__ dz) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float yv = y[i];
-  dz[i] = act ? dy[i] * (1.f - yv * yv) : dy[i];
+  const float yv = ldf(y, i), g = ldf(dy, i);
+  const float v = (int)(i % Cs) < K ? (act ? g * (1.f - yv * yv) : g) : 0.f;
+  dz[i] = (T)v;
 }
 
 // Partial bias gradient: block (b, j) sums pixels [b*chunk, (b+1)*chunk) of
 // channels j*256 .. j*256+255.
-__global__ void k_db_partial(const float* __restrict__ dz, long P, int Cs, int chunk, float* __restrict__ part) {
+template <typename T>
+__global__ void k_db_partial(const T* __restrict__ dz, long P, int Cs, int chunk, float* __restrict__ part) {
   const int c = blockIdx.y * blockDim.x + threadIdx.x;
   if (c >= Cs) return;
   const long p0 = (long)blockIdx.x * chunk;
   const long p1 = p0 + chunk < P ? p0 + chunk : P;
   float s = 0.f;
-  for (long p = p0; p < p1; ++p) s += dz[p * Cs + c];
+  for (long p = p0; p < p1; ++p) s += ldf(dz, p * Cs + c);
   part[(long)blockIdx.x * Cs + c] = s;
 }
 
@@ -39,11 +54,120 @@ __global__ void k_reduce_rows(const float* __restrict__ part, int nb, int n, flo
   out[i] = s;
 }
 
-// Partial weight gradient.  grid = (ceil(K/16), ceil(C/16) * taps, nb); block
-// 16 x 16 threads, thread (tk, tc) owns dW for (k0+tk, c0+tc) at one tap and
-// sums its block's pixel range.  Pixels go through LDS 64 at a time.
-__global__ __launch_bounds__(256) void k_dw_partial(const float* __restrict__ x, int Win, int Csx,
-                                                    const float* __restrict__ dz, int Ho, int Wo, int Csk, int K,
+// Weight gradient of one tap on MFMA.  grid = (taps, chunks), one wave per
+// 16-channel tile of K (KT waves).  The pixels are walked in segments of 64
+// output pixels along a row; a segment's dz (64 x Csk) and tap-shifted input
+// (64 x Csx) go to LDS as float32, channel pitch 16 KT / 16 CT (the pad
+// channels stay zero), while the next segment travels in registers.  Per 4
+// pixels each wave issues CT v_mfma_f32_16x16x4_f32: A[m][r] = dz[p_r][k0+m],
+// B[r][n] = x[p_r + tap][16t+n].  The block's K x C sums go to part.
+template <typename TX, typename T, int KT, int CT>
+__global__ __launch_bounds__(64 * KT) void k_dw_mfma(const TX* __restrict__ x, int Win, int Csx,
+                                                     const T* __restrict__ dz, int Ho, int Wo, int Csk, int KW,
+                                                     int taps, int segs_per_row, long nseg, long spc,
+                                                     float* __restrict__ part) {
+  constexpr int KP = 16 * KT, CP = 16 * CT, NTH = 64 * KT, SEG = 64;
+  constexpr int VE = 16 / (int)sizeof(T), VX = 16 / (int)sizeof(TX);  // elements per 16-byte vector
+  constexpr int VMIN = VE < VX ? VE : VX;
+  constexpr int NVMAX = (SEG * (KP + CP) / VMIN + NTH - 1) / NTH;
+  __shared__ __align__(16) float zs[SEG * KP];
+  __shared__ __align__(16) float xs[SEG * CP];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tap = blockIdx.x, ti = tap / KW, tj = tap % KW;
+  const long s0 = (long)blockIdx.y * spc;
+  const long s1 = s0 + spc < nseg ? s0 + spc : nseg;
+  for (int i = tid; i < SEG * KP; i += NTH) zs[i] = 0.f;
+  for (int i = tid; i < SEG * CP; i += NTH) xs[i] = 0.f;
+  const int zvp = Csk / VE, xvp = Csx / VX;  // vectors per pixel
+  const int nzv = SEG * zvp, nv = nzv + SEG * xvp;
+  u32x4 reg[NVMAX];
+  auto load = [&](long s) {
+    const int ho = (int)(s / segs_per_row), wo0 = (int)(s % segs_per_row) * SEG;
+#pragma unroll
+    for (int i = 0; i < NVMAX; ++i) {
+      const int v = tid + i * NTH;
+      u32x4 r = {0u, 0u, 0u, 0u};
+      if (v < nzv) {
+        const int px = v / zvp, c = v % zvp;
+        if (wo0 + px < Wo) r = *reinterpret_cast<const u32x4*>(dz + ((long)ho * Wo + wo0 + px) * Csk + c * VE);
+      } else if (v < nv) {
+        const int v2 = v - nzv, px = v2 / xvp, c = v2 % xvp;
+        if (wo0 + px < Wo)
+          r = *reinterpret_cast<const u32x4*>(x + ((long)(ho + ti) * Win + wo0 + px + tj) * Csx + c * VX);
+      }
+      reg[i] = r;
+    }
+  };
+  // 16 bytes of float32 (4 values) or bfloat16 (8 values, two per dword, low
+  // half first) as float32 into LDS.
+  auto put = [](float* d, const u32x4& r, int ve) {
+    if (ve == 4) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) d[e] = __uint_as_float(r[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        d[2 * e] = __uint_as_float(r[e] << 16);
+        d[2 * e + 1] = __uint_as_float(r[e] & 0xffff0000u);
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < NVMAX; ++i) {
+      const int v = tid + i * NTH;
+      if (v >= nv) continue;
+      if (v < nzv) put(zs + (v / zvp) * KP + (v % zvp) * VE, reg[i], VE);
+      else put(xs + ((v - nzv) / xvp) * CP + ((v - nzv) % xvp) * VX, reg[i], VX);
+    }
+  };
+  f32x4 acc[CT];
+#pragma unroll
+  for (int t = 0; t < CT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int lrow = lane & 15, lk = lane >> 4, k0 = 16 * wave;
+  if (s0 < s1) load(s0);
+  for (long s = s0; s < s1; ++s) {
+    __syncthreads();  // every wave is done with the previous segment
+    store();
+    __syncthreads();
+    if (s + 1 < s1) load(s + 1);  // in flight during the MFMAs
+#pragma unroll 4
+    for (int st = 0; st < SEG / 4; ++st) {
+      const int p = 4 * st + lk;
+      const float a = zs[p * KP + k0 + lrow];
+#pragma unroll
+      for (int t = 0; t < CT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xs[p * CP + 16 * t + lrow], acc[t], 0, 0, 0);
+    }
+  }
+  float* out = part + ((long)blockIdx.y * taps + tap) * KP * CP;
+#pragma unroll
+  for (int t = 0; t < CT; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[(k0 + 4 * lk + e) * CP + 16 * t + lrow] = acc[t][e];
+}
+
+// dW[k][c][KH-1-i][KW-1-j] = sum over chunks of the (tap = i KW + j) partials.
+__global__ void k_dw_reduce(const float* __restrict__ part, int nch, int taps, int KP, int CP, int K, int C, int KH,
+                            int KW, float* __restrict__ dw) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)K * C * taps) return;
+  const int c = (int)(idx % C);
+  const int k = (int)((idx / C) % K);
+  const int tap = (int)(idx / ((long)C * K));
+  float s = 0.f;
+  for (int ch = 0; ch < nch; ++ch) s += part[(((long)ch * taps + tap) * KP + k) * CP + c];
+  const int i = tap / KW, j = tap % KW;
+  dw[(((long)k * C + c) * KH + (KH - 1 - i)) * KW + (KW - 1 - j)] = s;
+}
+
+// Partial weight gradient on the VALU, for channel counts beyond the MFMA
+// kernel's tiles.  grid = (ceil(K/16), ceil(C/16) * taps, nb); block 16 x 16
+// threads, thread (tk, tc) owns dW for (k0+tk, c0+tc) at one tap and sums its
+// block's pixel range.  Pixels go through LDS 64 at a time.
+template <typename TX, typename T>
+__global__ __launch_bounds__(256) void k_dw_partial(const TX* __restrict__ x, int Win, int Csx,
+                                                    const T* __restrict__ dz, int Ho, int Wo, int Csk, int K,
                                                     int C, int KH, int KW, long chunk, float* __restrict__ part) {
   const int tk = threadIdx.x & 15, tc = threadIdx.x >> 4;
   const int k0 = blockIdx.x * 16;
@@ -62,8 +186,8 @@ __global__ __launch_bounds__(256) void k_dw_partial(const float* __restrict__ x,
       float zv = 0.f, xv = 0.f;
       if (p < p1) {
         const int ho = (int)(p / Wo), wo = (int)(p % Wo);
-        if (k0 + e < K) zv = dz[p * Csk + k0 + e];
-        if (c0 + e < C) xv = x[((long)(ho + i) * Win + wo + j) * Csx + c0 + e];
+        if (k0 + e < K) zv = ldf(dz, p * Csk + k0 + e);
+        if (c0 + e < C) xv = ldf(x, ((long)(ho + i) * Win + wo + j) * Csx + c0 + e);
       }
       zs[pl][e] = zv;
       xs[pl][e] = xv;
@@ -78,15 +202,15 @@ __global__ __launch_bounds__(256) void k_dw_partial(const float* __restrict__ x,
     part[(long)blockIdx.z * K * C * KH * KW + (((long)k * C + c) * KH + (KH - 1 - i)) * KW + (KW - 1 - j)] = acc;
 }
 
-__global__ void k_pad_dz(const float* __restrict__ dz, int Ho, int Wo, int Cs, int py, int px,
-                         float* __restrict__ out) {
+template <typename T>
+__global__ void k_pad_dz(const T* __restrict__ dz, int Ho, int Wo, int Cs, int py, int px, T* __restrict__ out) {
   const int Hp = Ho + 2 * py, Wp = Wo + 2 * px;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)Hp * Wp * Cs) return;
   const int c = (int)(idx % Cs);
   const long p = idx / Cs;
   const int yy = (int)(p / Wp) - py, xx = (int)(p % Wp) - px;
-  out[idx] = (yy >= 0 && yy < Ho && xx >= 0 && xx < Wo) ? dz[((long)yy * Wo + xx) * Cs + c] : 0.f;
+  out[idx] = (yy >= 0 && yy < Ho && xx >= 0 && xx < Wo) ? dz[((long)yy * Wo + xx) * Cs + c] : (T)0.f;
 }
 
 // Wt[c][k][a][b] = W[k][c][KH-1-a][KW-1-b]
@@ -105,51 +229,138 @@ __global__ void k_transpose_flip(const float* __restrict__ w, int K, int C, int 
 
 using namespace ccs;
 
-extern "C" int ccs_conv2d_backward(ccs_handle_t h, const float* x, int H, int W, int C, const float* w,
-                                   const float* b, int K, int KH, int KW, int act, const float* dy, float* dx,
-                                   float* dw, float* db) {
-  CCS_CHECK(h && x && w && dy && dw, CCS_ERR_ARG, "ccs_conv2d_backward: NULL argument");
-  CCS_CHECK(C >= 1 && K >= 1 && KH >= 1 && KW >= 1 && H >= KH && W >= KW, CCS_ERR_SHAPE,
-            "ccs_conv2d_backward: bad shape");
+namespace {
+
+template <typename TX, typename T, int KT, int CT>
+void launch_dw_mfma(const TX* x, int W, int Csx, const T* z, int Ho, int Wo, int Csk, int KW, int taps,
+                    int segs_row, long nseg, long spc, int nch, float* part, hipStream_t st) {
+  k_dw_mfma<TX, T, KT, CT><<<dim3(taps, nch), 64 * KT, 0, st>>>(x, W, Csx, z, Ho, Wo, Csk, KW, taps, segs_row, nseg,
+                                                                spc, part);
+}
+
+// Tiles of the MFMA weight gradient: the smallest instantiated count >= n.
+int dw_tiles(int n) { return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : n <= 7 ? 7 : 0; }
+
+template <typename TX, typename T>
+bool dw_mfma(int KT, int CT, const TX* x, int W, int Csx, const T* z, int Ho, int Wo, int Csk, int KW, int taps,
+             int segs_row, long nseg, long spc, int nch, float* part, hipStream_t st) {
+#define CCS_DW(kt, ct)                                                                                      \
+  if (KT == kt && CT == ct) {                                                                               \
+    launch_dw_mfma<TX, T, kt, ct>(x, W, Csx, z, Ho, Wo, Csk, KW, taps, segs_row, nseg, spc, nch, part, st); \
+    return true;                                                                                            \
+  }
+  CCS_DW(7, 7) CCS_DW(7, 1) CCS_DW(4, 4) CCS_DW(4, 1) CCS_DW(1, 1) CCS_DW(2, 2) CCS_DW(2, 1) CCS_DW(1, 2)
+  CCS_DW(7, 4) CCS_DW(4, 7) CCS_DW(7, 2) CCS_DW(2, 7) CCS_DW(4, 2) CCS_DW(2, 4) CCS_DW(1, 4) CCS_DW(1, 7)
+#undef CCS_DW
+  return false;
+}
+
+constexpr int kDwSeg = 64;        // output pixels per segment of k_dw_mfma
+constexpr int kDwTargetBlocks = 640;
+
+// The backward of one layer from its input x (element type TX) and output y
+// (element type T of the precision); dy has y's shape and type, dx x's shape
+// and y's type.
+template <typename TX, typename T>
+int conv_backward(ccs_handle_t h, const TX* x, const T* y, int H, int W, int C, const float* w, int K, int KH,
+                  int KW, int act, const T* dy, T* dx, float* dw, float* db, int precision) {
   const int Ho = H - KH + 1, Wo = W - KW + 1;
   const int Csx = feature_stride(C), Csk = feature_stride(K);
   const long P = (long)Ho * Wo;
   const long nz = P * Csk;
-  const int nb = 64;
-  const long chunk = (P + nb - 1) / nb;
+  const int taps = KH * KW;
   const size_t wcount = (size_t)K * C * KH * KW;
   const int Hp = Ho + 2 * (KH - 1), Wp = Wo + 2 * (KW - 1);
-  // scratch: y/dz (nz) | partials (nb * max(wcount, Csk)) | padded dz | Wt
-  const size_t part_n = (size_t)nb * (wcount > (size_t)Csk ? wcount : (size_t)Csk);
+  const int KT = dw_tiles((K + 15) / 16), CT = dw_tiles((C + 15) / 16);
+  const bool mfma = KT && CT && Csk <= 16 * KT && Csx <= 16 * CT;
+  const int segs_row = (Wo + kDwSeg - 1) / kDwSeg;
+  const long nseg = (long)Ho * segs_row;
+  int nch = (int)std::min<long>(nseg, std::max(1, kDwTargetBlocks / taps));
+  const long spc = (nseg + nch - 1) / nch;
+  nch = (int)((nseg + spc - 1) / spc);
+  const int nb = 64;  // pixel blocks of the bias gradient (and of the VALU weight gradient)
+  const long chunk = (P + nb - 1) / nb;
+  size_t part_n = (size_t)nb * Csk;
+  part_n = std::max(part_n, mfma ? (size_t)nch * taps * 256 * KT * CT : (size_t)nb * wcount);
   const size_t pad_n = dx ? (size_t)Hp * Wp * Csk : 0;
-  const size_t need = sizeof(float) * (nz + part_n + pad_n + wcount);
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t need = al(sizeof(T) * nz) + al(sizeof(float) * part_n) + al(sizeof(T) * pad_n) + sizeof(float) * wcount;
   int rc = h->misc2.reserve(need, h->stream);
   if (rc) return rc;
-  float* z = (float*)h->misc2.ptr;
-  float* part = z + nz;
-  float* zpad = part + part_n;
-  float* wt = zpad + pad_n;
+  char* base = (char*)h->misc2.ptr;
+  T* z = (T*)base;
+  float* part = (float*)(base + al(sizeof(T) * nz));
+  T* zpad = (T*)((char*)part + al(sizeof(float) * part_n));
+  float* wt = (float*)((char*)zpad + al(sizeof(T) * pad_n));
 
-  if ((rc = ccs_conv2d_forward(h, x, CCS_LAYOUT_NHWC, H, W, C, w, b, K, KH, KW, act, CCS_FP32, z))) return rc;
-  k_dz<<<ceil_div(nz, 256), 256, 0, h->stream>>>(z, dy, nz, act, z);
+  k_dz<T><<<ceil_div(nz, 256), 256, 0, h->stream>>>(y, dy, nz, Csk, K, act, z);
   if (db) {
-    k_db_partial<<<dim3(nb, ceil_div(Csk, 256)), 256, 0, h->stream>>>(z, P, Csk, (int)chunk, part);
+    k_db_partial<T><<<dim3(nb, ceil_div(Csk, 256)), 256, 0, h->stream>>>(z, P, Csk, (int)chunk, part);
     k_reduce_rows<<<ceil_div(Csk, 256), 256, 0, h->stream>>>(part, nb, Csk, part + (size_t)nb * Csk);
     CCS_HIP(hipMemcpyAsync(db, part + (size_t)nb * Csk, sizeof(float) * K, hipMemcpyDeviceToDevice, h->stream));
   }
-  {
+  if (mfma) {
+    dw_mfma<TX, T>(KT, CT, x, W, Csx, z, Ho, Wo, Csk, KW, taps, segs_row, nseg, spc, nch, part, h->stream);
+    k_dw_reduce<<<ceil_div((long)wcount, 256), 256, 0, h->stream>>>(part, nch, taps, 16 * KT, 16 * CT, K, C, KH, KW,
+                                                                  dw);
+  } else {
     dim3 grid(ceil_div(K, 16), ceil_div(C, 16) * KH * KW, nb);
-    k_dw_partial<<<grid, 256, 0, h->stream>>>(x, W, Csx, z, Ho, Wo, Csk, K, C, KH, KW, chunk, part);
+    k_dw_partial<TX, T><<<grid, 256, 0, h->stream>>>(x, W, Csx, z, Ho, Wo, Csk, K, C, KH, KW, chunk, part);
     k_reduce_rows<<<ceil_div((long)wcount, 256), 256, 0, h->stream>>>(part, nb, (int)wcount, dw);
   }
   if (dx) {
-    k_pad_dz<<<ceil_div((long)Hp * Wp * Csk, 256), 256, 0, h->stream>>>(z, Ho, Wo, Csk, KH - 1, KW - 1, zpad);
+    k_pad_dz<T><<<ceil_div((long)Hp * Wp * Csk, 256), 256, 0, h->stream>>>(z, Ho, Wo, Csk, KH - 1, KW - 1, zpad);
     k_transpose_flip<<<ceil_div((long)wcount, 256), 256, 0, h->stream>>>(w, K, C, KH, KW, wt);
     CCS_LAUNCH_CHECK();
     // dx has C output channels from K input channels.
-    if ((rc = ccs_conv2d_forward(h, zpad, CCS_LAYOUT_NHWC, Hp, Wp, K, wt, nullptr, C, KH, KW, 0, CCS_FP32, dx)))
+    if ((rc = ccs_conv2d_forward(h, zpad, CCS_LAYOUT_NHWC, Hp, Wp, K, wt, nullptr, C, KH, KW, 0, precision, dx)))
       return rc;
   }
   CCS_LAUNCH_CHECK();
   return CCS_OK;
+}
+
+int check_backward(const void* x, const float* w, const void* dy, float* dw, int H, int W, int C, int K, int KH,
+                   int KW, const char* who) {
+  CCS_CHECK(x && w && dy && dw, CCS_ERR_ARG, "%s: NULL argument", who);
+  CCS_CHECK(C >= 1 && K >= 1 && KH >= 1 && KW >= 1 && H >= KH && W >= KW, CCS_ERR_SHAPE, "%s: bad shape", who);
+  CCS_CHECK(C <= 4096 && K <= 4096, CCS_ERR_UNSUPPORTED, "%s: too many channels", who);
+  return CCS_OK;
+}
+
+}  // namespace
+
+extern "C" int ccs_conv2d_backward_stored(ccs_handle_t h, const void* x, int x_precision, const void* y,
+                                          int precision, int H, int W, int C, const float* w, int K, int KH, int KW,
+                                          int act, const void* dy, void* dx, float* dw, float* db) {
+  CCS_CHECK(h && y, CCS_ERR_ARG, "ccs_conv2d_backward_stored: NULL argument");
+  int rc = check_backward(x, w, dy, dw, H, W, C, K, KH, KW, "ccs_conv2d_backward_stored");
+  if (rc) return rc;
+  CCS_CHECK((precision == CCS_FP32 || precision == CCS_BF16) && (x_precision == CCS_FP32 || x_precision == CCS_BF16),
+            CCS_ERR_ARG, "ccs_conv2d_backward_stored: bad precision");
+  CCS_CHECK(!(x_precision == CCS_BF16 && precision == CCS_FP32), CCS_ERR_UNSUPPORTED,
+            "ccs_conv2d_backward_stored: a bfloat16 input under a float32 layer");
+  if (precision == CCS_BF16 && x_precision == CCS_BF16)
+    return conv_backward<bf16, bf16>(h, (const bf16*)x, (const bf16*)y, H, W, C, w, K, KH, KW, act, (const bf16*)dy,
+                                     (bf16*)dx, dw, db, precision);
+  if (precision == CCS_BF16)  // float32 input (the image) under a bfloat16 layer
+    return conv_backward<float, bf16>(h, (const float*)x, (const bf16*)y, H, W, C, w, K, KH, KW, act,
+                                      (const bf16*)dy, (bf16*)dx, dw, db, precision);
+  return conv_backward<float, float>(h, (const float*)x, (const float*)y, H, W, C, w, K, KH, KW, act,
+                                     (const float*)dy, (float*)dx, dw, db, precision);
+}
+
+extern "C" int ccs_conv2d_backward(ccs_handle_t h, const float* x, int H, int W, int C, const float* w,
+                                   const float* b, int K, int KH, int KW, int act, const float* dy, float* dx,
+                                   float* dw, float* db) {
+  CCS_CHECK(h, CCS_ERR_ARG, "ccs_conv2d_backward: NULL argument");
+  int rc = check_backward(x, w, dy, dw, H, W, C, K, KH, KW, "ccs_conv2d_backward");
+  if (rc) return rc;
+  // The layer's output is recomputed from x into the handle's conv_y scratch,
+  // then the stored-output path runs.
+  const int Ho = H - KH + 1, Wo = W - KW + 1;
+  if ((rc = h->conv_y.reserve(sizeof(float) * (size_t)Ho * Wo * feature_stride(K), h->stream))) return rc;
+  float* y = (float*)h->conv_y.ptr;
+  if ((rc = ccs_conv2d_forward(h, x, CCS_LAYOUT_NHWC, H, W, C, w, b, K, KH, KW, act, CCS_FP32, y))) return rc;
+  return conv_backward<float, float>(h, x, y, H, W, C, w, K, KH, KW, act, dy, dx, dw, db, CCS_FP32);
 }

@@ -80,7 +80,7 @@ extern "C" int ccs_create(ccs_handle_t* handle, void* hip_stream) {
 
 extern "C" int ccs_destroy(ccs_handle_t h) {
   if (!h) return CCS_OK;
-  for (ccs::Scratch* s : {&h->conv_w, &h->crf_a, &h->crf_msg, &h->crf_small, &h->crf_sums, &h->misc, &h->misc2, &h->crf_nan})
+  for (ccs::Scratch* s : {&h->conv_w, &h->crf_a, &h->crf_msg, &h->crf_small, &h->crf_sums, &h->misc, &h->misc2, &h->crf_nan, &h->conv_y})
     s->release(h->stream);
   (void)hipStreamSynchronize(h->stream);
   delete h;

@@ -106,6 +106,22 @@ def conv2d_backward(h, x, w, b, dy, act=True):
     return dx, dw, db
 
 
+def conv2d_backward_stored(h, x, y, w, dy, act=True, want_dx=True):
+    """Adjoint from the layer's input x and activated output y (no recompute).
+    x: H x W x Cs (float32, or y's dtype), y and dy: Ho x Wo x Ks (float32 or
+    bfloat16); returns dx (x's shape, y's dtype, or None), dw, db (float32)."""
+    H, W, _ = x.shape
+    K, C, KH, KW = w.shape
+    xp = capi.BF16 if x.dtype == torch.bfloat16 else capi.FP32
+    yp = capi.BF16 if y.dtype == torch.bfloat16 else capi.FP32
+    dx = torch.empty((H, W, capi.feature_stride(C)), dtype=y.dtype, device="cuda") if want_dx else None
+    dw = torch.empty((K, C, KH, KW), dtype=torch.float32, device="cuda")
+    db = torch.empty((K,), dtype=torch.float32, device="cuda")
+    call("ccs_conv2d_backward_stored", h.h, ptr(x), xp, ptr(y), yp, H, W, C, ptr(w), K, KH, KW, int(act), ptr(dy),
+         ptr(dx), ptr(dw), ptr(db))
+    return dx, dw, db
+
+
 def correlation_volume(h, f0, f1, C, disp_min, disp_step, D, epilogue="neg_softmax", out=None):
     H, W, _ = f0.shape
     precision = "bf16" if f0.dtype == torch.bfloat16 else "fp32"
@@ -118,11 +134,13 @@ def correlation_volume(h, f0, f1, C, disp_min, disp_step, D, epilogue="neg_softm
 
 
 def correlation_volume_backward(h, f0, f1, C, disp_min, disp_step, D, G):
+    """df0, df1 in the feature maps' dtype (float32 or bfloat16)."""
     H, W, Cs = f0.shape
+    precision = "bf16" if f0.dtype == torch.bfloat16 else "fp32"
     df0 = torch.empty_like(f0)
     df1 = torch.empty_like(f1)
-    call("ccs_correlation_volume_backward", h.h, ptr(f0), ptr(f1), H, W, C, int(disp_min), int(disp_step), D,
-         ptr(G), ptr(df0), ptr(df1))
+    call("ccs_correlation_volume_backward", h.h, ptr(f0), ptr(f1), PRECISIONS[precision], H, W, C, int(disp_min),
+         int(disp_step), D, ptr(G), ptr(df0), ptr(df1))
     return df0, df1
 
 

@@ -114,6 +114,20 @@ int ccs_conv2d_backward(ccs_handle_t h, const float* x, int H, int W, int C, con
                         const float* b, int K, int KH, int KW, int act, const float* dy,
                         float* dx, float* dw, float* db);
 
+/* Adjoint of one layer from its kept (or recomputed) input x (H x W x stride(C),
+ * element type x_precision) and activated output y ((H-KH+1) x (W-KW+1) x
+ * stride(K), element type precision: CCS_FP32 float32, CCS_BF16 bfloat16);
+ * tanh' is 1 - y^2, so nothing is recomputed.  dy (y's shape) and dx (x's
+ * shape, may be NULL) have y's element type; dw (K x C x KH x KW) and db (K,
+ * may be NULL) are float32.  x_precision may be CCS_FP32 under a CCS_BF16
+ * layer (the first layer reads the float32 image).  The weight gradient runs on
+ * v_mfma_f32_16x16x4_f32, dx on the forward MFMA kernels, with fixed-order pixel
+ * reductions.  No reference counterpart (the reference has no convolution
+ * backward). */
+int ccs_conv2d_backward_stored(ccs_handle_t h, const void* x, int x_precision, const void* y,
+                               int precision, int H, int W, int C, const float* w, int K, int KH,
+                               int KW, int act, const void* dy, void* dx, float* dw, float* db);
+
 /* ---- correlation cost volume -------------------------------------------- */
 
 /* StereoCorrelation::forward static (stereo/operators/stereocorrelation.h:38-40,
@@ -146,11 +160,13 @@ int ccs_correlation_volume(ccs_handle_t h, const void* f0, const void* f1, int p
                            int H, int W, int C, int disp_min, int disp_step, int D,
                            int epilogue, float* out);
 
-/* Adjoint of the fast path on float32 feature maps: G = dLoss/dS (H x W x D,
- * gradient of the raw correlation) -> df0, df1 (H x W x stride(C)). */
-int ccs_correlation_volume_backward(ccs_handle_t h, const float* f0, const float* f1, int H,
-                                    int W, int C, int disp_min, int disp_step, int D,
-                                    const float* G, float* df0, float* df1);
+/* Adjoint of the fast path: G = dLoss/dS (H x W x D float32, gradient of the raw
+ * correlation) -> df0, df1 (H x W x stride(C)) in the feature maps' precision
+ * (float32 or bfloat16, accumulated in float32).  Each output gathers its
+ * disparity band; no atomics. */
+int ccs_correlation_volume_backward(ccs_handle_t h, const void* f0, const void* f1, int precision,
+                                    int H, int W, int C, int disp_min, int disp_step, int D,
+                                    const float* G, void* df0, void* df1);
 
 /* Cross entropy of softmax_D(S) at target labels (targets outside [0, D) ignored):
  * loss (device, 1 double) and dS = softmax - onehot (H x W x D). */

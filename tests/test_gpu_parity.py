@@ -15,6 +15,7 @@ import torch
 
 import oracle as O
 from cnn_crf_stereo_amd import capi, ops, synth
+from cnn_crf_stereo_amd.capi import call, ptr
 
 pytestmark = pytest.mark.gpu
 
@@ -126,10 +127,13 @@ def test_conv_first_layer_nchw(handle, precision, C):
     np.testing.assert_allclose(from_hwc(y, 100), ref, atol=tol, rtol=0)
 
 
-@pytest.mark.parametrize("C,K,k", [(8, 8, 3), (8, 8, 2), (100, 100, 2), (3, 8, 3)])
-def test_conv_backward(handle, C, K, k):
+@pytest.mark.parametrize("C,K,k,H,W", [(8, 8, 3, 10, 37), (8, 8, 2, 10, 37), (100, 100, 2, 10, 37),
+                                       (3, 8, 3, 10, 37), (100, 100, 3, 40, 150), (1, 100, 3, 12, 70),
+                                       (64, 64, 3, 9, 66), (20, 40, 3, 7, 20)])
+def test_conv_backward(handle, C, K, k, H, W):
+    """The adjoint against the oracle's; the weight gradient runs on MFMA
+    (several 64-pixel segments per row and per block at 40 x 150)."""
     rng = np.random.default_rng(30 + C + k)
-    H, W = 10, 37
     x = rng.uniform(-1, 1, size=(C, H, W)).astype(np.float32)
     w = glorot(rng, K, C, k, k)
     b = rng.uniform(-0.1, 0.1, size=K).astype(np.float32)
@@ -143,6 +147,41 @@ def test_conv_backward(handle, C, K, k):
     np.testing.assert_allclose(from_hwc(dx, C), rdx, atol=2e-4 * np.abs(rdx).max(), rtol=0)
     np.testing.assert_allclose(dw.cpu().numpy(), rdw, atol=2e-4 * np.abs(rdw).max(), rtol=0)
     np.testing.assert_allclose(db.cpu().numpy(), rdb, atol=2e-4 * np.abs(rdb).max(), rtol=0)
+
+
+@pytest.mark.parametrize("C,K,k,x_bf16", [(100, 100, 3, True), (100, 100, 2, True), (1, 100, 3, False)])
+def test_conv_backward_stored_bf16(handle, C, K, k, x_bf16):
+    """BF16 form of the adjoint from the kept output: against the oracle's
+    adjoint of the bf16-rounded operands (x, w, dy), to the bf16 rounding of
+    the kept output y, of dz and of dx (1e-2 relative L2)."""
+    rng = np.random.default_rng(35 + C + k)
+    H, W = 14, 90
+    x = rng.uniform(-1, 1, size=(C, H, W)).astype(np.float32)
+    w = glorot(rng, K, C, k, k)
+    b = rng.uniform(-0.1, 0.1, size=K).astype(np.float32)
+    xg = to_hwc(x, capi.feature_stride(C))
+    xin = xg.to(torch.bfloat16) if x_bf16 else xg
+    lay = ops.ConvLayer(handle, w, b, "bf16")
+    y = lay(handle, xin) if x_bf16 else torch.empty(0)
+    if not x_bf16:  # a float32 planar image under a bf16 layer, as the towers' first layer reads it
+        img = torch.as_tensor(x, device="cuda")
+        y = torch.empty((H - k + 1, W - k + 1, capi.feature_stride(K)), dtype=torch.bfloat16, device="cuda")
+        call("ccs_conv2d_forward", handle.h, ptr(img), capi.LAYOUT_NCHW, H, W, C, ptr(lay.w), ptr(lay.b), K, k, k,
+             1, capi.BF16, ptr(y))
+    Ho, Wo = H - k + 1, W - k + 1
+    dy = rng.normal(size=(K, Ho, Wo)).astype(np.float32)
+    dyg = to_hwc(dy, capi.feature_stride(K)).to(torch.bfloat16)
+    dx, dw, db = ops.conv2d_backward_stored(handle, xin, y, lay.w, dyg, True, want_dx=x_bf16)
+    xr = x if not x_bf16 else bf16_round(x)
+    rdx, rdw, rdb = O.conv2d_bwd(xr, bf16_round(w), b, bf16_round(dy), True)
+
+    def rel(a, r):
+        return np.linalg.norm(a - r) / np.linalg.norm(r)
+
+    assert rel(dw.cpu().numpy(), rdw) < 1e-2
+    assert rel(db.cpu().numpy(), rdb) < 1e-2
+    if x_bf16:
+        assert rel(from_hwc(dx, C), rdx) < 1e-2
 
 
 # ---------------------------------------------------------------- correlation
