@@ -141,13 +141,19 @@ __device__ __forceinline__ float local_min(const Vec<LPL>& x) {
   return m;
 }
 
-// True in a lane if any of its values is a NaN (a class test, which
-// -fno-honor-nans cannot fold away).
+// True in a lane if any of its values is a NaN.  Under -fno-honor-nans the
+// compiler may fold any floating-point NaN test (and an integer test of the
+// value's bits, which it turns into one) to false, so the bits pass through an
+// empty asm first: an integer the compiler knows nothing about.
 template <int LPL>
 __device__ __forceinline__ bool any_nan(const Vec<LPL>& x) {
   bool r = false;
 #pragma unroll
-  for (int j = 0; j < LPL; ++j) r |= __builtin_amdgcn_class(x.v[j], 3);  // signalling or quiet NaN
+  for (int j = 0; j < LPL; ++j) {
+    unsigned u = __float_as_uint(x.v[j]);
+    asm volatile("" : "+v"(u));
+    r |= (u & 0x7fffffffu) > 0x7f800000u;  // exponent all ones, mantissa not zero
+  }
   return r;
 }
 
