@@ -32,6 +32,7 @@ SIGNATURES = {
     "ccs_pad_normalize": (_I, [_P, _P, _I, _I, _I, _I, _P, _P]),
     "ccs_rgb_to_gray": (_I, [_P, _P, _I, _I, _P]),
     "ccs_conv2d_forward": (_I, [_P, _P, _I, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P]),
+    "ccs_conv2d_forward_noalloc": (_I, [_P, _P, _I, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P]),
     "ccs_conv_packed_bytes": (_SZ, [_I, _I, _I, _I, _I]),
     "ccs_conv_pack_weights": (_I, [_P, _P, _I, _I, _I, _I, _I, _P]),
     "ccs_conv_forward_packed": (_I, [_P, _P, _I, _I, _I, _I, _P, _P, _I, _I, _I, _I, _I, _P]),

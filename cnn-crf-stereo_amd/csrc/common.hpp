@@ -17,6 +17,10 @@ void set_error(const char* fmt, ...);
 // Records the message and returns the status (usable as `return fail(...)`).
 int fail(int status, const char* fmt, ...);
 
+// Resolves the current device's address of the fixed scratch array of
+// ccs_conv2d_forward_noalloc (csrc/conv.hip); ccs_create calls it.
+int noalloc_scratch_setup();
+
 #define CCS_HIP(expr)                                                                  \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \

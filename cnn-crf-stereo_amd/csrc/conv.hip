@@ -18,6 +18,8 @@
 //   and any shape the MFMA kernels do not cover.
 #include <cmath>
 
+#include <mutex>
+
 #include "common.hpp"
 
 namespace ccs {
@@ -701,6 +703,75 @@ extern "C" int ccs_conv_forward_packed(ccs_handle_t h, const void* x, int x_layo
   launch_mfma(s, precision, x, packed, b, act, y, h->stream);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
+}
+
+// ---- allocation-free variant ----------------------------------------------
+// The packed weights of ccs_conv2d_forward_noalloc live in one fixed array per
+// GPU (held until the process ends) instead of the handle's scratch.  The
+// array's name is a host-side stand-in, not a device address: host code uses
+// only the address hipGetSymbolAddress returns on each device.
+constexpr size_t kNoAllocScratchBytes = 512u << 10;  // the largest MFMA pack is 419,328 bytes
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves the current device's copy of `scratch` once (ccs_create calls this
+// for the handle's device; the entry point repeats it on a device first used
+// without a handle created there).
+int ccs::noalloc_scratch_setup() {
+  int dev = 0;
+  CCS_HIP(hipGetDevice(&dev));
+  CCS_CHECK(dev >= 0 && dev < kMaxDevices, CCS_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return CCS_OK;
+  void* p = nullptr;
+  CCS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  CCS_CHECK(p != nullptr, CCS_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  g_scratch_ptr[dev] = p;
+  return CCS_OK;
+}
+
+static int noalloc_scratch(void** out) {
+  int dev = 0;
+  CCS_HIP(hipGetDevice(&dev));
+  CCS_CHECK(dev >= 0 && dev < kMaxDevices, CCS_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    *out = g_scratch_ptr[dev];
+  }
+  if (*out) return CCS_OK;
+  int rc = noalloc_scratch_setup();
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  *out = g_scratch_ptr[dev];
+  return CCS_OK;
+}
+
+extern "C" int ccs_conv2d_forward_noalloc(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C,
+                                          const float* w, const float* b, int K, int KH, int KW, int act,
+                                          int precision, void* y) {
+  CCS_CHECK(h && w, CCS_ERR_ARG, "ccs_conv2d_forward_noalloc: NULL argument");
+  int rc = check_conv(x, x_layout, H, W, C, K, KH, KW, precision, y);
+  if (rc) return rc;
+  const ConvShape s{H, W, C, K, KH, KW};
+  if (x_layout == CCS_LAYOUT_NHWC && mfma_supported(s)) {
+    const size_t need = packed_bytes(K, C, KH, KW, precision);
+    CCS_CHECK(need <= kNoAllocScratchBytes, CCS_ERR_UNSUPPORTED,
+              "ccs_conv2d_forward_noalloc: %d -> %d %dx%d needs %zu bytes of packed weights, the fixed scratch "
+              "holds %zu",
+              C, K, KH, KW, need, kNoAllocScratchBytes);
+    void* p = nullptr;
+    if ((rc = noalloc_scratch(&p))) return rc;
+    if ((rc = ccs_conv_pack_weights(h, w, K, C, KH, KW, precision, p))) return rc;
+    launch_mfma(s, precision, x, p, b, act, y, h->stream);
+    CCS_LAUNCH_CHECK();
+    return CCS_OK;
+  }
+  CCS_CHECK(x_layout == CCS_LAYOUT_NHWC || C <= 8, CCS_ERR_UNSUPPORTED,
+            "ccs_conv2d_forward_noalloc: NCHW input is only accepted for image layers (C <= 8)");
+  return conv_generic(h, x, x_layout, H, W, C, w, b, K, KH, KW, act, precision, y);  // reads w directly
 }
 
 extern "C" int ccs_conv2d_forward(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C, const float* w,

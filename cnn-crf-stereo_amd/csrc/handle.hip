@@ -72,6 +72,8 @@ extern "C" int ccs_create(ccs_handle_t* handle, void* hip_stream) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return ccs::fail(CCS_ERR_HIP, "ccs_create: no HIP device: %s", hipGetErrorString(e));
+  int rc = ccs::noalloc_scratch_setup();
+  if (rc) return rc;
   ccs_handle_s* h = new ccs_handle_s();
   h->stream = (hipStream_t)hip_stream;
   *handle = h;

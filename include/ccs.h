@@ -99,6 +99,20 @@ int ccs_conv2d_forward(ccs_handle_t h, const void* x, int x_layout, int H, int W
                        const float* w, const float* b, int K, int KH, int KW, int act,
                        int precision, void* y);
 
+/* ccs_conv2d_forward without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The weights
+ * are packed into one fixed 512 KiB array per GPU, held until the process
+ * ends, instead of the handle's scratch; every MFMA shape fits (100 -> 100 3x3
+ * in FP32 packs to 419,328 bytes) and a larger pack returns
+ * CCS_ERR_UNSUPPORTED.  All calls on one GPU share that array, which keeps the
+ * last call's packed weights: calls that overlap on one GPU (other streams,
+ * other host threads) race.  Same arguments, outputs and kernels as
+ * ccs_conv2d_forward. */
+int ccs_conv2d_forward_noalloc(ccs_handle_t h, const void* x, int x_layout, int H, int W, int C,
+                               const float* w, const float* b, int K, int KH, int KW, int act,
+                               int precision, void* y);
+
 size_t ccs_conv_packed_bytes(int K, int C, int KH, int KW, int precision);
 int ccs_conv_pack_weights(ccs_handle_t h, const float* w, int K, int C, int KH, int KW,
                           int precision, void* packed);
