@@ -78,6 +78,7 @@ inline int feature_stride(int C) { return (C + 7) / 8 * 8; }
 // handle's scratch from these.
 constexpr int kMomentBlocks = 128;
 constexpr int kReduceBlocks = 256;
+constexpr int kXentBlocks = 4096;
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 

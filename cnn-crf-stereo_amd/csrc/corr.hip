@@ -12,7 +12,9 @@
 // the channels), parks it in LDS over the dead feature tiles, and the epilogue
 // gathers the band w' = w + o_k, applies -softmax over k and writes the
 // H x W x D volume with 16-byte stores, D contiguous floats per pixel.
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -271,6 +273,115 @@ __global__ void k_corr_volume_bwd(const T* __restrict__ f0, const T* __restrict_
   df1[idx] = (T)g1;
 }
 
+// Adjoint of the fast path on MFMA.  Both feature gradients have the form
+//   out[p][c] = sum_k Gt[p][k] F[p + e_k][c],  e_k = e0 + k estep,
+// df0: Gt = G, F = f1, e_k = o_k;  df1: Gt[p][k] = G[p - o_k][k], F = f0,
+// e_k = -o_k (each G entry is read once either way; terms whose source pixel
+// leaves the image are zero).  For a block of 64 output pixels of one row this
+// is a banded GEMM over the window of F columns: per chunk of KC disparities
+// the chunk's Gt (64 x KC) and F window (64 + span columns, all channels, as
+// float32) go to LDS, and each wave computes its 16 pixels x 16*CT channels
+// with v_mfma_f32_16x16x4_f32: A[m][r] = Gt[m][k] where window column r lies
+// at e_k from pixel m (0 off the band), B[r][n] = F[column r][16t+n].  The
+// band lookup k(d) of a column offset d is a small LDS table per chunk.  No
+// atomics: every output is written once by the block that owns it.
+constexpr int kBwdTW = 64;  // output pixels per block (four waves of 16)
+constexpr int kBwdKC = 64;  // disparities per chunk at |estep| = 1
+
+template <typename T, int CT>
+__global__ __launch_bounds__(256, 2) void k_corr_bwd_mfma(const T* __restrict__ fsrc, const float* __restrict__ G,
+                                                          int H, int W, int Cs, int D, int e0, int estep,
+                                                          int gather, int o0, int ostep, T* __restrict__ out) {
+  constexpr int CP = 16 * CT, TW = kBwdTW;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int astep = estep < 0 ? -estep : estep;
+  const int KC = max(1, kBwdKC / astep);
+  const int span_max = (KC - 1) * astep;
+  float* Gt = reinterpret_cast<float*>(lds);          // [TW][KC]
+  float* Fw = Gt + TW * KC;                           // [TW + span_max][CP]
+  int* ktab = reinterpret_cast<int*>(Fw + (TW + span_max) * CP);  // [TW + span_max + 16]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lm = lane & 15, lr = lane >> 4;
+  const int h = blockIdx.y, p0 = blockIdx.x * TW;
+  const long rowpix = (long)h * W;
+  const T* frow = fsrc + rowpix * Cs;
+  const float* grow = G + rowpix * D;
+  constexpr int VE = 16 / (int)sizeof(T);
+  const int vpp = Cs / VE;
+  // zero the pad channels of the window once (they are never written again)
+  for (int i = tid; i < (TW + span_max) * CP; i += 256) Fw[i] = 0.f;
+  f32x4 acc[CT];
+#pragma unroll
+  for (int t = 0; t < CT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int kc = 0; kc < D; kc += KC) {
+    const int kn = min(KC, D - kc);
+    const int span = (kn - 1) * astep;
+    const int emin = estep > 0 ? e0 + kc * estep : e0 + (kc + kn - 1) * estep;
+    const int win = TW + span;
+    __syncthreads();  // the previous chunk's MFMAs are done with Gt, Fw, ktab
+    // Gt[p][kk] of this chunk
+    for (int i = tid; i < TW * KC; i += 256) {
+      const int p = i / KC, kk = i % KC;
+      float v = 0.f;
+      if (kk < kn && p0 + p < W) {
+        const int k = kc + kk;
+        const int src = gather ? p0 + p - (o0 + k * ostep) : p0 + p;
+        if (src >= 0 && src < W) v = grow[(long)src * D + k];
+      }
+      Gt[i] = v;
+    }
+    // F window: columns p0 + emin .. p0 + emin + win - 1, zero outside the image
+    for (int v = tid; v < win * vpp; v += 256) {
+      const int px = v / vpp, g = v % vpp, col = p0 + emin + px;
+      u32x4 r = {0u, 0u, 0u, 0u};
+      if (col >= 0 && col < W) r = *reinterpret_cast<const u32x4*>(frow + (long)col * Cs + g * VE);
+      float* d = Fw + px * CP + g * VE;
+      if constexpr (VE == 4) {
+        *reinterpret_cast<u32x4*>(d) = r;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          d[2 * e] = __uint_as_float(r[e] << 16);
+          d[2 * e + 1] = __uint_as_float(r[e] & 0xffff0000u);
+        }
+      }
+    }
+    // ktab[d + 15]: the chunk-local disparity at column offset d from a pixel
+    // (window column index minus the pixel's index), or -1 off the band.
+    const int dbase = emin - e0 - kc * estep;  // offset of window column 0 relative to e_kc
+    for (int i = tid; i < win + 16; i += 256) {
+      const int dd = dbase + (i - 15);
+      int kk = -1;
+      if (dd % estep == 0) {
+        const int q = dd / estep;
+        if (q >= 0 && q < kn) kk = q;
+      }
+      ktab[i] = kk;
+    }
+    __syncthreads();
+    // wave tile: pixels 16 wave + m, window columns 16 wave + j, j < 16 + span
+    const int steps = (16 + span + 3) / 4;
+    for (int st = 0; st < steps; ++st) {
+      const int j = 16 * wave + 4 * st + lr;  // window column of this lane's A entry
+      const int kk = ktab[4 * st + lr - lm + 15];
+      const float av = kk >= 0 ? Gt[(16 * wave + lm) * KC + kk] : 0.f;
+      const bool jin = j < win;
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        const float bv = jin ? Fw[j * CP + 16 * t + lm] : 0.f;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < CT; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int p = p0 + 16 * wave + 4 * lr + e, c = 16 * t + lm;
+      if (p < W && c < Cs) out[(rowpix + p) * Cs + c] = (T)acc[t][e];
+    }
+}
+
 // Cross entropy of softmax_D(S) at the target label; dS = softmax - onehot.
 __global__ void k_softmax_xent(const float* __restrict__ S, long HW, int D, const int* __restrict__ target,
                                float* __restrict__ dS, double* __restrict__ part) {
@@ -299,12 +410,18 @@ __global__ void k_softmax_xent(const float* __restrict__ S, long HW, int D, cons
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// Deterministic sum of n doubles (one block of 256 threads, a fixed tree).
 __global__ void k_sum_f64_corr(const double* __restrict__ in, int n, double* __restrict__ out) {
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += in[i];
-    out[0] = s;
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += in[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 // LDS bytes of one workgroup: the feature tiles, later overlaid by S, then the statistics.
@@ -457,14 +574,48 @@ extern "C" int ccs_correlation_volume_backward(ccs_handle_t h, const void* f0, c
   CCS_CHECK(precision == CCS_FP32 || precision == CCS_BF16, CCS_ERR_ARG, "ccs_correlation_volume_backward: precision");
   const int Cs = feature_stride(C);
   const long total = (long)H * W * Cs;
-  if (precision == CCS_BF16)
+  const int o0 = -disp_min, ostep = -disp_step;
+  if (Cs <= 112 && ostep != 0) {
+    // MFMA form: two launches (df0 from f1, df1 from f0 with G gathered along the band)
+    const int astep = ostep < 0 ? -ostep : ostep;
+    const int KC = std::max(1, kBwdKC / astep);
+    const int span_max = (KC - 1) * astep;
+    const int ct = (Cs + 15) / 16;
+    const int CT = ct <= 1 ? 1 : ct <= 2 ? 2 : ct <= 4 ? 4 : 7;  // the instantiated channel tiles
+    const size_t lds = sizeof(float) * ((size_t)kBwdTW * KC + (size_t)(kBwdTW + span_max) * 16 * CT) +
+                       sizeof(int) * (kBwdTW + span_max + 16);
+    const dim3 grid(ceil_div(W, kBwdTW), H);
+    auto launch = [&](auto tag, const void* fs, int e0, int es, int gather, void* o) {
+      using T = decltype(tag);
+      auto go = [&](auto ct) {
+        constexpr int NT = decltype(ct)::value;
+        auto fn = k_corr_bwd_mfma<T, NT>;
+        if (lds > 64 * 1024) (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        fn<<<grid, 256, lds, h->stream>>>((const T*)fs, G, H, W, Cs, D, e0, es, gather, o0, ostep, (T*)o);
+      };
+      if (CT == 1) go(std::integral_constant<int, 1>{});
+      else if (CT == 2) go(std::integral_constant<int, 2>{});
+      else if (CT == 4) go(std::integral_constant<int, 4>{});
+      else go(std::integral_constant<int, 7>{});
+    };
+    if (precision == CCS_BF16) {
+      launch(bf16{}, f1, o0, ostep, 0, df0);
+      launch(bf16{}, f0, -o0, -ostep, 1, df1);
+    } else {
+      launch(float{}, f1, o0, ostep, 0, df0);
+      launch(float{}, f0, -o0, -ostep, 1, df1);
+    }
+    CCS_LAUNCH_CHECK();
+    return CCS_OK;
+  }
+  if (precision == CCS_BF16)  // VALU form for channel counts beyond the MFMA tiles
     k_corr_volume_bwd<bf16><<<ceil_div(total, 256), 256, 0, h->stream>>>((const bf16*)f0, (const bf16*)f1, H, W, C,
-                                                                        Cs, -disp_min, -disp_step, D, G, (bf16*)df0,
+                                                                        Cs, o0, ostep, D, G, (bf16*)df0,
                                                                         (bf16*)df1);
   else
     k_corr_volume_bwd<float><<<ceil_div(total, 256), 256, 0, h->stream>>>((const float*)f0, (const float*)f1, H, W,
-                                                                         C, Cs, -disp_min, -disp_step, D, G,
-                                                                         (float*)df0, (float*)df1);
+                                                                         C, Cs, o0, ostep, D, G, (float*)df0,
+                                                                         (float*)df1);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }
@@ -472,12 +623,13 @@ extern "C" int ccs_correlation_volume_backward(ccs_handle_t h, const void* f0, c
 extern "C" int ccs_softmax_xent(ccs_handle_t h, const float* S, int HW, int D, const int* target, double* loss,
                                 float* dS) {
   CCS_CHECK(h && S && target && loss && dS && HW >= 1 && D >= 1, CCS_ERR_ARG, "ccs_softmax_xent: bad argument");
-  const int nb = kReduceBlocks;
+  // One wave per pixel at a time: enough blocks for every wave slot of the chip.
+  const int nb = (int)std::min<long>(kXentBlocks, ((long)HW + 3) / 4);
   int rc;
   if ((rc = h->misc2.reserve(sizeof(double) * nb, h->stream))) return rc;
   double* part = (double*)h->misc2.ptr;
   k_softmax_xent<<<nb, 256, 0, h->stream>>>(S, HW, D, target, dS, part);
-  k_sum_f64_corr<<<1, 64, 0, h->stream>>>(part, nb, loss);
+  k_sum_f64_corr<<<1, 256, 0, h->stream>>>(part, nb, loss);
   CCS_LAUNCH_CHECK();
   return CCS_OK;
 }

@@ -22,28 +22,41 @@ __device__ __forceinline__ float ldf(const T* p, long i) {
   return (float)p[i];
 }
 
-// dz = dy (1 - y^2) (act) or dy, for the K live channels; 0 in the pad channels.
+// dz = dy (1 - y^2) (act) or dy for the K live channels, 0 in the pad
+// channels, and the bias-gradient partials: block b takes pixels
+// [b chunk, (b+1) chunk), thread c channel c, and writes the sum of its stored
+// (rounded) dz values to part[c][b] (channel-major, for k_reduce_cols).
 template <typename T>
-__global__ void k_dz(const T* __restrict__ y, const T* __restrict__ dy, long n, int Cs, int K, int act,
-                     T* __restrict__ dz) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float yv = ldf(y, i), g = ldf(dy, i);
-  const float v = (int)(i % Cs) < K ? (act ? g * (1.f - yv * yv) : g) : 0.f;
-  dz[i] = (T)v;
-}
-
-// Partial bias gradient: block (b, j) sums pixels [b*chunk, (b+1)*chunk) of
-// channels j*256 .. j*256+255.
-template <typename T>
-__global__ void k_db_partial(const T* __restrict__ dz, long P, int Cs, int chunk, float* __restrict__ part) {
-  const int c = blockIdx.y * blockDim.x + threadIdx.x;
-  if (c >= Cs) return;
+__global__ void k_dz_db(const T* __restrict__ y, const T* __restrict__ dy, long P, int Cs, int K, int act, int chunk,
+                        T* __restrict__ dz, float* __restrict__ part) {
   const long p0 = (long)blockIdx.x * chunk;
   const long p1 = p0 + chunk < P ? p0 + chunk : P;
+  for (int c = threadIdx.x; c < Cs; c += blockDim.x) {
+    float s = 0.f;
+    for (long p = p0; p < p1; ++p) {
+      const long i = p * Cs + c;
+      const float yv = ldf(y, i), g = ldf(dy, i);
+      const T t = (T)(c < K ? (act ? g * (1.f - yv * yv) : g) : 0.f);
+      dz[i] = t;
+      s += (float)t;
+    }
+    part[(long)c * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// out[c] = sum of the n partials part[c][0..n) (one block per c, a fixed tree).
+__global__ void k_reduce_cols(const float* __restrict__ part, int n, float* __restrict__ out) {
+  __shared__ float sh[256];
+  const float* row = part + (long)blockIdx.x * n;
   float s = 0.f;
-  for (long p = p0; p < p1; ++p) s += ldf(dz, p * Cs + c);
-  part[(long)blockIdx.x * Cs + c] = s;
+  for (int i = threadIdx.x; i < n; i += 256) s += row[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
 __global__ void k_reduce_rows(const float* __restrict__ part, int nb, int n, float* __restrict__ out) {
@@ -54,18 +67,22 @@ __global__ void k_reduce_rows(const float* __restrict__ part, int nb, int n, flo
   out[i] = s;
 }
 
-// Weight gradient of one tap on MFMA.  grid = (taps, chunks), one wave per
-// 16-channel tile of K (KT waves).  The pixels are walked in segments of 64
-// output pixels along a row; a segment's dz (64 x Csk) and tap-shifted input
-// (64 x Csx) go to LDS as float32, channel pitch 16 KT / 16 CT (the pad
-// channels stay zero), while the next segment travels in registers.  Per 4
-// pixels each wave issues CT v_mfma_f32_16x16x4_f32: A[m][r] = dz[p_r][k0+m],
-// B[r][n] = x[p_r + tap][16t+n].  The block's K x C sums go to part.
+// Weight gradient of one tap on MFMA.  A 1-D grid of taps x chunks blocks,
+// one wave per 16-channel tile of K (KT waves).  Blocks b and b + 8 share an
+// XCD (and its L2), so a chunk's taps are given to consecutive blocks of one
+// XCD: they read the same dz and overlapping inputs at about the same time.
+// The pixels are walked in segments of 64 output pixels along a row; a
+// segment's dz (64 x Csk) and tap-shifted input (64 x Csx) go to LDS as
+// float32, channel pitch 16 KT / 16 CT (the pad channels stay zero), while the
+// next segment travels in registers.  Per 4 pixels each wave issues CT
+// v_mfma_f32_16x16x4_f32: A[m][r] = dz[p_r][k0+m], B[r][n] = x[p_r + tap][16t+n].
+// The block's K x C sums go to part[chunk][tap].  Register budget: four waves
+// per SIMD, so two 7-wave blocks share a CU (LDS: 2 x 57 KB for 7 x 7 tiles).
 template <typename TX, typename T, int KT, int CT>
-__global__ __launch_bounds__(64 * KT) void k_dw_mfma(const TX* __restrict__ x, int Win, int Csx,
-                                                     const T* __restrict__ dz, int Ho, int Wo, int Csk, int KW,
-                                                     int taps, int segs_per_row, long nseg, long spc,
-                                                     float* __restrict__ part) {
+__global__ __launch_bounds__(64 * KT, 4) void k_dw_mfma(const TX* __restrict__ x, int Win, int Csx,
+                                                        const T* __restrict__ dz, int Ho, int Wo, int Csk, int KW,
+                                                        int taps, int nch, int segs_per_row, long nseg, long spc,
+                                                        float* __restrict__ part) {
   constexpr int KP = 16 * KT, CP = 16 * CT, NTH = 64 * KT, SEG = 64;
   constexpr int VE = 16 / (int)sizeof(T), VX = 16 / (int)sizeof(TX);  // elements per 16-byte vector
   constexpr int VMIN = VE < VX ? VE : VX;
@@ -73,28 +90,50 @@ __global__ __launch_bounds__(64 * KT) void k_dw_mfma(const TX* __restrict__ x, i
   __shared__ __align__(16) float zs[SEG * KP];
   __shared__ __align__(16) float xs[SEG * CP];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int tap = blockIdx.x, ti = tap / KW, tj = tap % KW;
-  const long s0 = (long)blockIdx.y * spc;
+  // (tap, chunk) of this block: taps of one chunk on consecutive blocks of one XCD.
+  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
+  const int tap = local % taps, chunk = (local / taps) * 8 + xcd;
+  if (chunk >= nch) return;  // the grid is rounded up to whole groups of 8 chunks
+  const int ti = tap / KW, tj = tap % KW;
+  const long s0 = (long)chunk * spc;
   const long s1 = s0 + spc < nseg ? s0 + spc : nseg;
   for (int i = tid; i < SEG * KP; i += NTH) zs[i] = 0.f;
   for (int i = tid; i < SEG * CP; i += NTH) xs[i] = 0.f;
   const int zvp = Csk / VE, xvp = Csx / VX;  // vectors per pixel
   const int nzv = SEG * zvp, nv = nzv + SEG * xvp;
+  // Per vector slot of this thread (independent of the segment): the pixel in
+  // the segment, the element offset within the pixel, the LDS offset and the
+  // operand (0: dz, 1: x, -1: none).
+  int vpx[NVMAX], voff[NVMAX], vlds[NVMAX], vkind[NVMAX];
+#pragma unroll
+  for (int i = 0; i < NVMAX; ++i) {
+    const int v = tid + i * NTH;
+    if (v < nzv) {
+      vpx[i] = v / zvp;
+      voff[i] = (v % zvp) * VE;
+      vlds[i] = vpx[i] * KP + voff[i];
+      vkind[i] = 0;
+    } else if (v < nv) {
+      vpx[i] = (v - nzv) / xvp;
+      voff[i] = ((v - nzv) % xvp) * VX;
+      vlds[i] = vpx[i] * CP + voff[i];
+      vkind[i] = 1;
+    } else {
+      vpx[i] = voff[i] = vlds[i] = 0;
+      vkind[i] = -1;
+    }
+  }
   u32x4 reg[NVMAX];
   auto load = [&](long s) {
     const int ho = (int)(s / segs_per_row), wo0 = (int)(s % segs_per_row) * SEG;
+    const T* zrow = dz + ((long)ho * Wo + wo0) * Csk;
+    const TX* xrow = x + ((long)(ho + ti) * Win + wo0 + tj) * Csx;
 #pragma unroll
     for (int i = 0; i < NVMAX; ++i) {
-      const int v = tid + i * NTH;
+      const u32x4* src = vkind[i] == 0 ? reinterpret_cast<const u32x4*>(zrow + (long)vpx[i] * Csk + voff[i])
+                                       : reinterpret_cast<const u32x4*>(xrow + (long)vpx[i] * Csx + voff[i]);
       u32x4 r = {0u, 0u, 0u, 0u};
-      if (v < nzv) {
-        const int px = v / zvp, c = v % zvp;
-        if (wo0 + px < Wo) r = *reinterpret_cast<const u32x4*>(dz + ((long)ho * Wo + wo0 + px) * Csk + c * VE);
-      } else if (v < nv) {
-        const int v2 = v - nzv, px = v2 / xvp, c = v2 % xvp;
-        if (wo0 + px < Wo)
-          r = *reinterpret_cast<const u32x4*>(x + ((long)(ho + ti) * Win + wo0 + px + tj) * Csx + c * VX);
-      }
+      if (vkind[i] >= 0 && wo0 + vpx[i] < Wo) r = *src;
       reg[i] = r;
     }
   };
@@ -102,23 +141,19 @@ __global__ __launch_bounds__(64 * KT) void k_dw_mfma(const TX* __restrict__ x, i
   // half first) as float32 into LDS.
   auto put = [](float* d, const u32x4& r, int ve) {
     if (ve == 4) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) d[e] = __uint_as_float(r[e]);
+      *reinterpret_cast<u32x4*>(d) = r;
     } else {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        d[2 * e] = __uint_as_float(r[e] << 16);
-        d[2 * e + 1] = __uint_as_float(r[e] & 0xffff0000u);
-      }
+      const u32x4 lo = {r[0] << 16, r[0] & 0xffff0000u, r[1] << 16, r[1] & 0xffff0000u};
+      const u32x4 hi = {r[2] << 16, r[2] & 0xffff0000u, r[3] << 16, r[3] & 0xffff0000u};
+      reinterpret_cast<u32x4*>(d)[0] = lo;
+      reinterpret_cast<u32x4*>(d)[1] = hi;
     }
   };
   auto store = [&]() {
 #pragma unroll
     for (int i = 0; i < NVMAX; ++i) {
-      const int v = tid + i * NTH;
-      if (v >= nv) continue;
-      if (v < nzv) put(zs + (v / zvp) * KP + (v % zvp) * VE, reg[i], VE);
-      else put(xs + ((v - nzv) / xvp) * CP + ((v - nzv) % xvp) * VX, reg[i], VX);
+      if (vkind[i] == 0) put(zs + vlds[i], reg[i], VE);
+      else if (vkind[i] == 1) put(xs + vlds[i], reg[i], VX);
     }
   };
   f32x4 acc[CT];
@@ -140,7 +175,7 @@ __global__ __launch_bounds__(64 * KT) void k_dw_mfma(const TX* __restrict__ x, i
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xs[p * CP + 16 * t + lrow], acc[t], 0, 0, 0);
     }
   }
-  float* out = part + ((long)blockIdx.y * taps + tap) * KP * CP;
+  float* out = part + ((long)chunk * taps + tap) * KP * CP;
 #pragma unroll
   for (int t = 0; t < CT; ++t)
 #pragma unroll
@@ -234,8 +269,9 @@ namespace {
 template <typename TX, typename T, int KT, int CT>
 void launch_dw_mfma(const TX* x, int W, int Csx, const T* z, int Ho, int Wo, int Csk, int KW, int taps,
                     int segs_row, long nseg, long spc, int nch, float* part, hipStream_t st) {
-  k_dw_mfma<TX, T, KT, CT><<<dim3(taps, nch), 64 * KT, 0, st>>>(x, W, Csx, z, Ho, Wo, Csk, KW, taps, segs_row, nseg,
-                                                                spc, part);
+  const unsigned blocks = (unsigned)(8 * taps * ((nch + 7) / 8));  // whole groups of 8 chunks, one per XCD
+  k_dw_mfma<TX, T, KT, CT><<<blocks, 64 * KT, 0, st>>>(x, W, Csx, z, Ho, Wo, Csk, KW, taps, nch, segs_row, nseg, spc,
+                                                      part);
 }
 
 // Tiles of the MFMA weight gradient: the smallest instantiated count >= n.
@@ -255,8 +291,8 @@ bool dw_mfma(int KT, int CT, const TX* x, int W, int Csx, const T* z, int Ho, in
   return false;
 }
 
-constexpr int kDwSeg = 64;        // output pixels per segment of k_dw_mfma
-constexpr int kDwTargetBlocks = 640;
+constexpr int kDwSeg = 64;            // output pixels per segment of k_dw_mfma
+constexpr int kDwTargetBlocks = 512;  // two resident blocks per CU of the 256: one round
 
 // The backward of one layer from its input x (element type TX) and output y
 // (element type T of the precision); dy has y's shape and type, dx x's shape
@@ -275,12 +311,14 @@ int conv_backward(ccs_handle_t h, const TX* x, const T* y, int H, int W, int C, 
   const bool mfma = KT && CT && Csk <= 16 * KT && Csx <= 16 * CT;
   const int segs_row = (Wo + kDwSeg - 1) / kDwSeg;
   const long nseg = (long)Ho * segs_row;
-  int nch = (int)std::min<long>(nseg, std::max(1, kDwTargetBlocks / taps));
+  int nch = (int)std::min<long>(nseg, std::max(8, kDwTargetBlocks / taps / 8 * 8));
   const long spc = (nseg + nch - 1) / nch;
   nch = (int)((nseg + spc - 1) / spc);
-  const int nb = 64;  // pixel blocks of the bias gradient (and of the VALU weight gradient)
+  const int nb = 64;  // pixel blocks of the VALU weight gradient
   const long chunk = (P + nb - 1) / nb;
-  size_t part_n = (size_t)nb * Csk;
+  const int dz_chunk = 64;  // pixels per block of the bias-gradient partials
+  const int nbz = (int)((P + dz_chunk - 1) / dz_chunk);
+  size_t part_n = (size_t)nbz * Csk + Csk;
   part_n = std::max(part_n, mfma ? (size_t)nch * taps * 256 * KT * CT : (size_t)nb * wcount);
   const size_t pad_n = dx ? (size_t)Hp * Wp * Csk : 0;
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
@@ -293,11 +331,11 @@ int conv_backward(ccs_handle_t h, const TX* x, const T* y, int H, int W, int C, 
   T* zpad = (T*)((char*)part + al(sizeof(float) * part_n));
   float* wt = (float*)((char*)zpad + al(sizeof(T) * pad_n));
 
-  k_dz<T><<<ceil_div(nz, 256), 256, 0, h->stream>>>(y, dy, nz, Csk, K, act, z);
+  k_dz_db<T><<<nbz, Csk <= 256 ? (Csk + 63) / 64 * 64 : 256, 0, h->stream>>>(y, dy, P, Csk, K, act, dz_chunk, z,
+                                                                                part);
   if (db) {
-    k_db_partial<T><<<dim3(nb, ceil_div(Csk, 256)), 256, 0, h->stream>>>(z, P, Csk, (int)chunk, part);
-    k_reduce_rows<<<ceil_div(Csk, 256), 256, 0, h->stream>>>(part, nb, Csk, part + (size_t)nb * Csk);
-    CCS_HIP(hipMemcpyAsync(db, part + (size_t)nb * Csk, sizeof(float) * K, hipMemcpyDeviceToDevice, h->stream));
+    k_reduce_cols<<<Csk, 256, 0, h->stream>>>(part, nbz, part + (size_t)nbz * Csk);
+    CCS_HIP(hipMemcpyAsync(db, part + (size_t)nbz * Csk, sizeof(float) * K, hipMemcpyDeviceToDevice, h->stream));
   }
   if (mfma) {
     dw_mfma<TX, T>(KT, CT, x, W, Csx, z, Ho, Wo, Csk, KW, taps, segs_row, nseg, spc, nch, part, h->stream);

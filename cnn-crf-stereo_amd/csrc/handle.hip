@@ -119,7 +119,8 @@ extern "C" int ccs_reserve(ccs_handle_t h, int H, int W, int D, int C, int iters
   const size_t misc = mx(mx(sizeof(int) * (size_t)D, sizeof(double) * (2 * 5 * (size_t)ccs::kMomentBlocks + 10)),
                          sizeof(int) * hw + 2 * sizeof(double));
   // misc2: Gaussian temporary of the edge weights, float64 reduction partials.
-  const size_t misc2 = mx(sizeof(float) * hw, 2 * sizeof(double) * (size_t)ccs::kReduceBlocks);
+  const size_t misc2 = mx(mx(sizeof(float) * hw, 2 * sizeof(double) * (size_t)ccs::kReduceBlocks),
+                          sizeof(double) * (size_t)ccs::kXentBlocks);
   int rc;
   if (packed && (rc = h->conv_w.reserve(packed, h->stream))) return rc;
   if ((rc = h->misc.reserve(misc, h->stream))) return rc;
