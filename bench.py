@@ -177,6 +177,33 @@ def time_form(pkg, h, data, spec, params, precision, steps, warmup, side=None):
     return ms, summary, out
 
 
+def time_crf_forms(h, data, spec, cost, steps):
+    """ms per SlackProp run (all iterations, rows + columns + decode) on the
+    frame's cost volume with edge weights (ccs_slackprop: w L1, w L2) and with
+    the same penalties given per edge (ccs_slackprop_pw), on the handle's stream."""
+    import torch
+    from cnn_crf_stereo_amd import ops
+
+    left = torch.as_tensor(data["left"], device="cuda")
+    wx, wy = ops.edge_weights(h, left[0], 0.7, 10.0, 1.0, 1.0)
+    L1, L2, iters = 0.5, 2.0, spec["iters"]
+    pw = torch.stack([wx * L1, wy * L1, wx * L2, wy * L2]).contiguous()
+    pen, _ = ops.pairwise_penalties(h, pw)
+    out = {}
+    for name, fn in (("edge_weights", lambda: ops.slackprop(h, cost, wx, wy, L1, L2, 2, iters)),
+                     ("per_edge", lambda: ops.slackprop_pw(h, cost, pen, 2, iters))):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = round(e0.elapsed_time(e1) / steps, 4)
+    out["per_edge_over_edge_weights"] = round(out["per_edge"] / out["edge_weights"], 4)
+    return out
+
+
 def time_pipelined(pkg, lanes, data, spec, params, precision, steps, warmup):
     """Steady-state ms per frame with len(lanes) frames in flight: frame i runs
     entirely on lane i % len(lanes), a (handle, stream) pair with its own CRF
@@ -441,6 +468,7 @@ def main():
         side = (capi.Handle(s2), s2)
     ms_bf16, launches_bf16, out_bf16 = time_form(pkg, h, data, spec, params, "bf16", args.steps, args.warmup, side)
     result_extra = {"pcie_inclusive_ms_per_frame": round(launches_bf16.pop("_pcie_inclusive_ms"), 4)}
+    result_extra["crf_ms_per_run"] = time_crf_forms(h, data, spec, out_bf16["cost"], max(3, args.steps // 2))
     if side is not None:
         # Per-launch figures come from a one-stream pass: with the towers on two
         # streams each event interval also spans the other tower's launches.

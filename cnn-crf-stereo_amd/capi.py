@@ -49,6 +49,9 @@ SIGNATURES = {
                                               ctypes.POINTER(_F)]),
     "ccs_slackprop": (_I, [_P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _I, _P, _P, _P, _P]),
     "ccs_slackprop_status": (_I, [_P]),
+    "ccs_slackprop_forward_pw": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, ctypes.POINTER(_F)]),
+    "ccs_pairwise_penalties": (_I, [_P, _P, _I, _I, _I, _P, _P]),
+    "ccs_slackprop_pw": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
     "ccs_slackprop_decode_gap": (_I, [_P, _P, _P, _I, _I, _I, _F, _F, _I, _P, _P]),
     "ccs_slackprop_sweep": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _F, _F, _I, _I, _P, _P, _P]),
     "ccs_crf_energy": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _F, _F, _I, _P]),

@@ -166,6 +166,32 @@ __global__ void k_sum_pairs(const double* __restrict__ part, int nb, double* __r
   }
 }
 
+// Per-edge penalties: the pairwise volume pw (2 levels planes of H x W, the
+// channels in the reference's order P1x, P1y, P2x, P2y, ...) re-laid out as the
+// reference does before handing it to its solver (stereo/operators/
+// slackprop.cpp:88-92): split the channels into (2, levels) with x/y fastest,
+// flip the levels, and move x/y last: pen[l][y][x][c] = pw[c + 2 (levels-1-l)][y][x].
+// Block b also writes bad[b] = 1 if one of its edges has P1 < 0 or P2 < P1
+// (the min-sum form needs 0 <= P1 <= P2; P1 = pen[levels-1], P2 = pen[0]).
+__global__ void k_pen_relayout(const float* __restrict__ pw, int levels, int H, int W, float* __restrict__ pen,
+                               int* __restrict__ bad) {
+  const long HW = (long)H * W;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;  // pixel
+  bool wrong = false;
+  if (i < HW) {
+    const int x = (int)(i % W), y = (int)(i / W);
+    for (int l = 0; l < levels; ++l)
+      for (int c = 0; c < 2; ++c) pen[((long)l * HW + i) * 2 + c] = pw[(long)(c + 2 * (levels - 1 - l)) * HW + i];
+    for (int c = 0; c < 2; ++c) {
+      const bool edge = c == 0 ? x + 1 < W : y + 1 < H;
+      const float p1 = pw[(long)c * HW + i], p2 = pw[(long)(c + 2 * (levels - 1)) * HW + i];
+      if (edge && !(p1 >= 0.f && p2 >= p1)) wrong = true;  // also catches NaN
+    }
+  }
+  const int any = __syncthreads_or(wrong);
+  if (threadIdx.x == 0) bad[blockIdx.x] = any;
+}
+
 }  // namespace ccs
 
 using namespace ccs;
@@ -180,18 +206,21 @@ static int check_crf_args(int H, int W, int D, float L1, float L2, int delta) {
   return CCS_OK;
 }
 
-extern "C" int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, const float* wx, const float* wy,
-                                   int H, int W, int D, float L1, float L2, int delta, int direction, int* labels,
-                                   double* chain_sum, double* kept_sum) {
+// One half-step.  pen == nullptr: edge weights wx / wy with L1, L2; otherwise
+// per-edge penalties pen (levels x H x W x 2: P1 = pen[levels-1], P2 = pen[0],
+// x components for the row edges, y for the column edges) and wx, wy unused.
+static int sweep(ccs_handle_t h, const float* cost, float* A, const float* wx, const float* wy, const float* pen,
+                 int levels, int H, int W, int D, float L1, float L2, int delta, int direction, int* labels,
+                 double* chain_sum, double* kept_sum) {
   CCS_CHECK(h && A, CCS_ERR_ARG, "ccs_slackprop_sweep: NULL argument");
-  int rc = check_crf_args(H, W, D, L1, L2, delta);
+  int rc = check_crf_args(H, W, D, pen ? 0.f : L1, pen ? 0.f : L2, delta);
   if (rc) return rc;
   CCS_CHECK(direction >= 0 && direction <= 3, CCS_ERR_ARG, "ccs_slackprop_sweep: direction must be 0, 1, 2 or 3");
-  CCS_CHECK((direction != 0 && direction != 3) || wx, CCS_ERR_ARG, "ccs_slackprop_sweep: rows need wx");
+  CCS_CHECK(pen || (direction != 0 && direction != 3) || wx, CCS_ERR_ARG, "ccs_slackprop_sweep: rows need wx");
   CCS_CHECK(direction != 3 || (cost && cost != A), CCS_ERR_ARG,
             "ccs_slackprop_sweep: the first row sweep reads cost, which must not be A");
-  CCS_CHECK(direction != 1 || (wy && cost), CCS_ERR_ARG, "ccs_slackprop_sweep: columns need cost and wy");
-  CCS_CHECK(direction != 2 || (wx && labels), CCS_ERR_ARG, "ccs_slackprop_sweep: decode needs wx and labels");
+  CCS_CHECK(direction != 1 || ((pen || wy) && cost), CCS_ERR_ARG, "ccs_slackprop_sweep: columns need cost and wy");
+  CCS_CHECK(direction != 2 || ((pen || wx) && labels), CCS_ERR_ARG, "ccs_slackprop_sweep: decode needs wx and labels");
   const size_t vol = (size_t)H * W * D;
   if ((rc = h->crf_msg.reserve(vol * sizeof(float), h->stream))) return rc;
   const int nmax = H > W ? H : W;
@@ -212,6 +241,13 @@ extern "C" int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, 
   } else {
     a.w = wx; a.pcs = W; a.pns = 1; a.n_chains = H; a.n = W;
   }
+  if (pen) {
+    const long plane = 2 * (long)H * W;
+    const int comp = direction == 1 ? 1 : 0;  // y components couple the columns
+    a.w = pen + (long)(levels - 1) * plane + comp;
+    a.w2 = pen + comp;
+    a.wmul = 2;
+  }
   if (direction == 0 || direction == 3) rc = launch_chain(kRowsMinorant, a, h->stream);
   else if (direction == 1) rc = launch_chain(kColsMinorant, a, h->stream);
   else {
@@ -225,11 +261,18 @@ extern "C" int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, 
   return CCS_OK;
 }
 
-extern "C" int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H, int W,
-                             int D, float L1, float L2, int delta, int iters, int* labels, float* unaries_out,
-                             double* sums, double* lower_bound) {
-  CCS_CHECK(h && cost && wx && wy && labels, CCS_ERR_ARG, "ccs_slackprop: NULL argument");
-  int rc = check_crf_args(H, W, D, L1, L2, delta);
+extern "C" int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, const float* wx, const float* wy,
+                                   int H, int W, int D, float L1, float L2, int delta, int direction, int* labels,
+                                   double* chain_sum, double* kept_sum) {
+  return sweep(h, cost, A, wx, wy, nullptr, 0, H, W, D, L1, L2, delta, direction, labels, chain_sum, kept_sum);
+}
+
+// The whole run (ccs_slackprop, or ccs_slackprop_pw with pen).
+static int run(ccs_handle_t h, const float* cost, const float* wx, const float* wy, const float* pen, int levels,
+               int H, int W, int D, float L1, float L2, int delta, int iters, int* labels, float* unaries_out,
+               double* sums, double* lower_bound) {
+  CCS_CHECK(h && cost && (pen || (wx && wy)) && labels, CCS_ERR_ARG, "ccs_slackprop: NULL argument");
+  int rc = check_crf_args(H, W, D, pen ? 0.f : L1, pen ? 0.f : L2, delta);
   if (rc) return rc;
   CCS_CHECK(iters >= 0 && iters <= 10000, CCS_ERR_ARG, "ccs_slackprop: bad iteration count %d", iters);
   const size_t vol = (size_t)H * W * D;
@@ -244,14 +287,14 @@ extern "C" int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx,
   }
   for (int it = 0; it < iters; ++it) {
     // The first row sweep reads the unaries from cost (A = cost is never copied).
-    if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, it == 0 ? 3 : 0, nullptr, S + 2 * it,
-                                  K + 2 * it)))
+    if ((rc = sweep(h, cost, A, wx, wy, pen, levels, H, W, D, L1, L2, delta, it == 0 ? 3 : 0, nullptr, S + 2 * it,
+                    K + 2 * it)))
       return rc;
-    if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, 1, nullptr, S + 2 * it + 1,
-                                  K + 2 * it + 1)))
+    if ((rc = sweep(h, cost, A, wx, wy, pen, levels, H, W, D, L1, L2, delta, 1, nullptr, S + 2 * it + 1,
+                    K + 2 * it + 1)))
       return rc;
   }
-  if ((rc = ccs_slackprop_sweep(h, cost, A, wx, wy, H, W, D, L1, L2, delta, 2, labels, S + 2 * iters, nullptr)))
+  if ((rc = sweep(h, cost, A, wx, wy, pen, levels, H, W, D, L1, L2, delta, 2, labels, S + 2 * iters, nullptr)))
     return rc;
   if (lower_bound) k_bound<<<1, 1, 0, h->stream>>>(S, iters, lower_bound);
   if (sums)
@@ -259,6 +302,74 @@ extern "C" int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx,
   if (unaries_out)
     CCS_HIP(hipMemcpyAsync(unaries_out, A, vol * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
   CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx, const float* wy, int H, int W,
+                             int D, float L1, float L2, int delta, int iters, int* labels, float* unaries_out,
+                             double* sums, double* lower_bound) {
+  CCS_CHECK(h && wx && wy, CCS_ERR_ARG, "ccs_slackprop: NULL argument");
+  return run(h, cost, wx, wy, nullptr, 0, H, W, D, L1, L2, delta, iters, labels, unaries_out, sums, lower_bound);
+}
+
+extern "C" int ccs_slackprop_pw(ccs_handle_t h, const float* cost, const float* pen, int levels, int H, int W, int D,
+                                int delta, int iters, int* labels, float* unaries_out, double* sums,
+                                double* lower_bound) {
+  CCS_CHECK(h && pen, CCS_ERR_ARG, "ccs_slackprop_pw: NULL argument");
+  CCS_CHECK(levels == 1 || levels == 2, CCS_ERR_UNSUPPORTED,
+            "ccs_slackprop_pw: %d penalty levels (1: Potts, 2: P1 / P2 are implemented)", levels);
+  return run(h, cost, nullptr, nullptr, pen, levels, H, W, D, 0.f, 0.f, delta, iters, labels, unaries_out, sums,
+             lower_bound);
+}
+
+extern "C" int ccs_pairwise_penalties(ccs_handle_t h, const float* pw, int pw_channels, int H, int W, float* pen,
+                                      int* bad) {
+  CCS_CHECK(h && pw && pen && bad, CCS_ERR_ARG, "ccs_pairwise_penalties: NULL argument");
+  CCS_CHECK(pw_channels == 2 || pw_channels == 4, CCS_ERR_UNSUPPORTED,
+            "ccs_pairwise_penalties: %d channels (2: Potts, 4: P1 / P2 are implemented)", pw_channels);
+  CCS_CHECK(H >= 1 && W >= 1, CCS_ERR_ARG, "ccs_pairwise_penalties: empty image");
+  const long HW = (long)H * W;
+  k_pen_relayout<<<ceil_div(HW, 256), 256, 0, h->stream>>>(pw, pw_channels / 2, H, W, pen, bad);
+  CCS_LAUNCH_CHECK();
+  return CCS_OK;
+}
+
+extern "C" int ccs_slackprop_forward_pw(ccs_handle_t h, const float* cost_volume, const float* pw_volume,
+                                        float* output, int in, int ic, int ih, int iw, int pw_channels, int max_iter,
+                                        float* lower_bound) {
+  CCS_CHECK(in == 1, CCS_ERR_UNSUPPORTED, "ccs_slackprop_forward_pw: only one sample (in == 1) is supported");
+  CCS_CHECK(h && cost_volume && pw_volume && output, CCS_ERR_ARG, "ccs_slackprop_forward_pw: NULL argument");
+  CCS_CHECK(pw_channels == 2 || pw_channels == 4, CCS_ERR_UNSUPPORTED,
+            "ccs_slackprop_forward_pw: %d pairwise channels (2: Potts, 4: P1 / P2 are implemented)", pw_channels);
+  const size_t hw = (size_t)ih * iw;
+  const int nbad = (int)ceil_div((long)hw, 256);
+  // misc: labels | bound | per-block validation flags | penalties (levels x H x W x 2)
+  const size_t off_lb = (sizeof(int) * hw + 7) / 8 * 8;
+  const size_t off_bad = off_lb + sizeof(double);
+  const size_t off_pen = (off_bad + sizeof(int) * nbad + 255) / 256 * 256;
+  int rc;
+  if ((rc = h->misc.reserve(off_pen + sizeof(float) * hw * pw_channels, h->stream))) return rc;
+  char* base = (char*)h->misc.ptr;
+  int* lab = (int*)base;
+  double* lb = (double*)(base + off_lb);
+  int* bad = (int*)(base + off_bad);
+  float* pen = (float*)(base + off_pen);
+  if ((rc = ccs_pairwise_penalties(h, pw_volume, pw_channels, ih, iw, pen, bad))) return rc;
+  // delta = 2: the second level costs jumps of two labels or more (P1 for one label).
+  if ((rc = ccs_slackprop_pw(h, cost_volume, pen, pw_channels / 2, ih, iw, ic, 2, max_iter, lab, nullptr, nullptr, lb)))
+    return rc;
+  // labels are written as float into the caller's output (stereo/operators/slackprop.cpp:103).
+  if ((rc = ccs_remap(h, lab, ih, iw, 0.f, 1.f, output))) return rc;
+  double lbh = 0.0;
+  std::vector<int> badh(nbad);
+  CCS_HIP(hipMemcpyAsync(&lbh, lb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  CCS_HIP(hipMemcpyAsync(badh.data(), bad, sizeof(int) * nbad, hipMemcpyDeviceToHost, h->stream));
+  if ((rc = ccs_slackprop_status(h))) return rc;  // synchronises
+  for (int b = 0; b < nbad; ++b)
+    CCS_CHECK(!badh[b], CCS_ERR_ARG,
+              "ccs_slackprop_forward_pw: the pairwise volume needs 0 <= P1 <= P2 on every edge (near pixel %d)",
+              b * 256);
+  if (lower_bound) *lower_bound = (float)lbh;
   return CCS_OK;
 }
 

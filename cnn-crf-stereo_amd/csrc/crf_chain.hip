@@ -236,13 +236,15 @@ __device__ __forceinline__ Vec<LPL> near_max(const Vec<LPL>& x, int delta, Shift
   return acc;
 }
 
-// m(s) = min(U(s), near(U)(s) + w L1, min U + w L2).
-template <int LPL, int DC>
-__device__ __forceinline__ Vec<LPL> minconv(const Vec<LPL>& U, float umin, float w, float L1, float L2,
+// m(s) = min(U(s), near(U)(s) + P1, min U + P2) with the edge's penalties
+// P1 = w L1, P2 = w L2 (PE false: w is the edge weight) or P1 = w, P2 = w2 (PE:
+// per-edge penalties).
+template <int LPL, int DC, bool PE = false>
+__device__ __forceinline__ Vec<LPL> minconv(const Vec<LPL>& U, float umin, float w, float w2, float L1, float L2,
                                             int delta, ShiftCarry& c) {
   const Vec<LPL> nr = near_min<LPL, DC>(U, delta, c);
-  const float far = umin + w * L2;
-  const float wl1 = w * L1;
+  const float far = PE ? umin + w2 : umin + w * L2;
+  const float wl1 = PE ? w : w * L1;
   Vec<LPL> m;
 #pragma unroll
   for (int j = 0; j < LPL; ++j) m.v[j] = fminf(fminf(U.v[j], nr.v[j] + wl1), far);
@@ -421,7 +423,7 @@ __device__ double chain_kept(const float* v, const double* b, int n, int lane, d
 // of registers and the load for node s + P is issued right after slot s is read
 // (clamped to the last node, so it never leaves the chain).  The last partial
 // block is an unrolled tail that issues no loads.
-template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK>
+template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK, bool PE>
 __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
   constexpr bool kCols = MODE == kColsMinorant;
@@ -438,7 +440,8 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   const float* __restrict__ Ab = (kCols ? a.A : a.A_in) + pbase * D;
   float* __restrict__ Aw = a.A + pbase * D;
   const float* __restrict__ Cb = kCols ? a.cost + pbase * D : nullptr;
-  const float* __restrict__ wb = a.w + pbase;
+  const float* __restrict__ wb = a.w + pbase * a.wmul;
+  const float* __restrict__ wb2 = PE ? a.w2 + pbase * a.wmul : nullptr;
   float* __restrict__ msg = a.msg + (long)c * n * D;
   bool valid[LPL];
 #pragma unroll
@@ -451,12 +454,17 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     return th;
   };
 
-  // The chain's edge weights go to LDS once (the sweeps read them a block of
-  // steps at a time, one per lane).  u_lds takes the per-node minima of a pass
-  // (float64 sums are formed after it).
+  // The chain's edge weights (per-edge P1 and P2 with PE) go to LDS once (the
+  // sweeps read them a block of steps at a time, one per lane).  u_lds takes
+  // the per-node minima of a pass (float64 sums are formed after it).
   float* w_lds = reinterpret_cast<float*>(b_lds + n);
   float* u_lds = w_lds + n;
-  for (int i = lane; i < n - 1; i += 64) w_lds[i] = wb[(long)i * a.pns];
+  float* w2_lds = u_lds + n;  // PE only
+  const long wstep = a.pns * a.wmul;
+  for (int i = lane; i < n - 1; i += 64) {
+    w_lds[i] = wb[(long)i * wstep];
+    if constexpr (PE) w2_lds[i] = wb2[(long)i * wstep];
+  }
   __syncthreads();
 
   Vec<LPL> ra[P], rc[P], rb[P];
@@ -481,6 +489,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   };
   // weights of steps s0 .. s0+63 of the backward pass (edge n-2-s)
   auto wlane_b = [&](int s0) { return w_lds[max(n - 2 - (s0 + lane), 0)]; };
+  auto wlane_b2 = [&](int s0) { return PE ? w2_lds[max(n - 2 - (s0 + lane), 0)] : 0.f; };
   Vec<LPL> bh;
 #pragma unroll
   for (int j = 0; j < LPL; ++j) bh.v[j] = 0.f;
@@ -491,10 +500,10 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     for (int k = 0; k < LPL; ++k) U.v[k] = th.v[k] + bh.v[k];
     return U;
   };
-  auto step_b = [&](int s, const Vec<LPL>& U, float wi) {
+  auto step_b = [&](int s, const Vec<LPL>& U, float wi, float wi2) {
     const int i = n - 2 - s;
     const float u = wave_min(local_min(U));
-    const Vec<LPL> m = minconv<LPL, DC>(U, u, wi, L1v, L2v, a.delta, cc);
+    const Vec<LPL> m = minconv<LPL, DC, PE>(U, u, wi, wi2, L1v, L2v, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) bh.v[k] = valid[k] ? m.v[k] - u : 0.f;
     if constexpr (kCols) {
@@ -511,9 +520,9 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     for (int j = 0; j < P; ++j) load_b(j, ra[j], rc[j]);
   }
   int s0 = 0;
-  float wv = wlane_b(0);
+  float wv = wlane_b(0), wv2 = wlane_b2(0);
   for (; s0 + P <= nb; s0 += P) {
-    const float wnext = wlane_b(s0 + P);
+    const float wnext = wlane_b(s0 + P), wnext2 = wlane_b2(s0 + P);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
@@ -522,15 +531,16 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       const Vec<LPL> U = plus_bh(theta_of(ra[j], rc[j]));
       __builtin_amdgcn_sched_barrier(kSchedValu);
       load_b(s0 + j + P, ra[j], rc[j]);
-      step_b(s0 + j, U, lane_weight(wv, j));
+      step_b(s0 + j, U, lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f);
     }
     wv = wnext;
+    wv2 = wnext2;
   }
 #pragma unroll
   for (int j = 0; j < P; ++j)
     if (s0 + j < nb) {
       if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
-      step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])), lane_weight(wv, j));
+      step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])), lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f);
     }
   const double bsum = chain_offsets(u_lds, b_lds, n, lane);  // b_0 = sum of all minima
 
@@ -565,13 +575,14 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     };
     // weights of steps i0 .. i0+63 of the forward pass (edge i)
     auto wlane_f = [&](int i0) { return w_lds[max(min(i0 + lane, n - 2), 0)]; };
+    auto wlane_f2 = [&](int i0) { return PE ? w2_lds[max(min(i0 + lane, n - 2), 0)] : 0.f; };
     Vec<LPL> R = th0;
     Vec<LPL> a_cur = a0;
     // The float64 kept constant (the chain minimum this pass leaves behind) is
     // formed from the per-node minima v_i after the pass: chain_kept().
     // step_f takes RB = R + Bh_i, A_{i+1} and theta_{i+1}.
     // step_f returns the node's new unaries; the caller stores them.
-    auto step_f = [&](int i, const Vec<LPL>& RB, const Vec<LPL>& an, const Vec<LPL>& thn, float wi) {
+    auto step_f = [&](int i, const Vec<LPL>& RB, const Vec<LPL>& an, const Vec<LPL>& thn, float wi, float wi2) {
       Vec<LPL> gt, V;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) {
@@ -580,12 +591,12 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       }
       const float v = wave_min(local_min(V));
       u_lds[i] = v;  // every lane writes the same value
-      const Vec<LPL> m = minconv<LPL, DC>(V, v, wi, L1v, L2v, a.delta, cc);
+      const Vec<LPL> m = minconv<LPL, DC, PE>(V, v, wi, wi2, L1v, L2v, a.delta, cc);
       Vec<LPL> mm;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) mm.v[k] = valid[k] ? m.v[k] : -kInfF;
       const Vec<LPL> nmax = near_max<LPL, DC>(mm, a.delta, cc);
-      const float wl1 = wi * L1v;
+      const float wl1 = PE ? wi : wi * L1v;
       Vec<LPL> out;
 #pragma unroll
       for (int k = 0; k < LPL; ++k) {
@@ -614,7 +625,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       constexpr int K = seg_len<LPL>();
       struct Seg {
         Vec<LPL> a[K], c[K], m;
-        float u, w;
+        float u, w, w2;
       };
       Seg X, Y;
       Vec<LPL> bs[K];
@@ -636,6 +647,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         const int e = max(min(i0 + lane, n - 2), 0);
         B.u = u_lds[e];
         B.w = w_lds[e];
+        B.w2 = PE ? w2_lds[e] : 0.f;
       };
       // The segment of cnt (1 .. K) steps from i0, its data in B.
       auto run_seg = [&](int i0, int cnt, const Seg& B) {
@@ -645,7 +657,8 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
           if (t < cnt) {
             const Vec<LPL> U = plus(theta_of(B.a[t], B.c[t]), Xb);
             const float u = lane_weight(B.u, t);
-            const Vec<LPL> m = minconv<LPL, DC>(U, u, lane_weight(B.w, t), L1v, L2v, a.delta, cc);
+            const Vec<LPL> m = minconv<LPL, DC, PE>(U, u, lane_weight(B.w, t), PE ? lane_weight(B.w2, t) : 0.f,
+                                                    L1v, L2v, a.delta, cc);
 #pragma unroll
             for (int k = 0; k < LPL; ++k) Xb.v[k] = valid[k] ? m.v[k] - u : 0.f;
             bs[t] = Xb;
@@ -656,7 +669,8 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         for (int t = 0; t < K; ++t)
           if (t < cnt)
             st_buf<LPL, FULL>(sW, t * strideB, lane, D,
-                              step_f(i0 + t, plus(R, bs[t]), B.a[t], theta_of(B.a[t], B.c[t]), lane_weight(B.w, t)));
+                              step_f(i0 + t, plus(R, bs[t]), B.a[t], theta_of(B.a[t], B.c[t]), lane_weight(B.w, t),
+                                     PE ? lane_weight(B.w2, t) : 0.f));
       };
       // Two segments per iteration, alternating between X and Y; the next
       // segment's loads are issued before the current segment's steps.
@@ -684,9 +698,9 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       for (int j = 0; j < P; ++j) load_f(j, rb[j], ra[j], rc[j]);
     }
     int i0 = 0;
-    float wf = wlane_f(0);
+    float wf = wlane_f(0), wf2 = wlane_f2(0);
     for (; i0 + P <= nb; i0 += P) {
-      const float wnext = wlane_f(i0 + P);
+      const float wnext = wlane_f(i0 + P), wnext2 = wlane_f2(i0 + P);
 #pragma unroll
       for (int j = 0; j < P; ++j) {
         // A_{i+1} lives on as the next step's a_cur: copied out of the slot.
@@ -699,13 +713,16 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         }
         __builtin_amdgcn_sched_barrier(kSchedValu);
         load_f(i0 + j + P, rb[j], ra[j], rc[j]);
-        store_f(i0 + j, step_f(i0 + j, RB, an, thn, lane_weight(wf, j)));
+        store_f(i0 + j, step_f(i0 + j, RB, an, thn, lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f));
       }
       wf = wnext;
+      wf2 = wnext2;
     }
 #pragma unroll
     for (int j = 0; j < P; ++j)
-      if (i0 + j < nb) store_f(i0 + j, step_f(i0 + j, plus(R, rb[j]), ra[j], theta_of(ra[j], rc[j]), lane_weight(wf, j)));
+      if (i0 + j < nb)
+        store_f(i0 + j, step_f(i0 + j, plus(R, rb[j]), ra[j], theta_of(ra[j], rc[j]), lane_weight(wf, j),
+                               PE ? lane_weight(wf2, j) : 0.f));
     }
     {
       Vec<LPL> out;
@@ -736,9 +753,10 @@ __device__ double chain_total(const float* u, int n, int lane) {
 // first + dir, ..., with the unaries (and, with MSG, the message buffer) P nodes
 // ahead in a register ring, as in k_chain, and the edge weights w (in LDS, n - 1
 // of them) one block ahead in lanes.
-template <int LPL, bool FULL, int P, bool MSG, typename Step>
-__device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const float* w_lds, int n, int D,
-                                          int lane, int first, int cnt, int dir, int aoff, int woff, Step&& step) {
+template <int LPL, bool FULL, int P, bool MSG, bool PE, typename Step>
+__device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const float* w_lds, const float* w2_lds,
+                                          int n, int D, int lane, int first, int cnt, int dir, int aoff, int woff,
+                                          Step&& step) {
   static_assert(P <= 64, "one weight per lane and step of a block");
   if (cnt <= 0) return;
   Vec<LPL> ra[P], rm[P];
@@ -748,12 +766,13 @@ __device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const f
     if constexpr (MSG) mv = ld_buf<LPL, FULL>(rM, i * D * 4, lane, D, 0.f);
   };
   auto wlane = [&](int s0) { return w_lds[max(min(first + dir * (s0 + lane) + woff, n - 2), 0)]; };
+  auto wlane2 = [&](int s0) { return PE ? w2_lds[max(min(first + dir * (s0 + lane) + woff, n - 2), 0)] : 0.f; };
 #pragma unroll
   for (int j = 0; j < P; ++j) load(j, ra[j], rm[j]);
   int s0 = 0;
-  float wv = wlane(0);
+  float wv = wlane(0), wv2 = wlane2(0);
   for (; s0 + P <= cnt; s0 += P) {
-    const float wnext = wlane(s0 + P);
+    const float wnext = wlane(s0 + P), wnext2 = wlane2(s0 + P);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const Vec<LPL> av = fresh(ra[j]);
@@ -761,13 +780,14 @@ __device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const f
       if constexpr (MSG) mv = fresh(rm[j]);
       __builtin_amdgcn_sched_barrier(kSchedValu);
       load(s0 + j + P, ra[j], rm[j]);
-      step(first + dir * (s0 + j), av, mv, lane_weight(wv, j));
+      step(first + dir * (s0 + j), av, mv, lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f);
     }
     wv = wnext;
+    wv2 = wnext2;
   }
 #pragma unroll
   for (int j = 0; j < P; ++j)
-    if (s0 + j < cnt) step(first + dir * (s0 + j), ra[j], rm[j], lane_weight(wv, j));
+    if (s0 + j < cnt) step(first + dir * (s0 + j), ra[j], rm[j], lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f);
 }
 
 // Row decode with both message passes at once: wave 0 runs the forward
@@ -777,7 +797,7 @@ __device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const f
 // sit in the buffer (wave 0 the right half, wave 1 the left half).  F_i + Bh_i
 // and the decision rule are those of the single-wave decode, so the labels are
 // the same; the chain latency is n steps instead of 2n.
-template <int LPL, int P, bool FULL, int DC, bool GAP>
+template <int LPL, int P, bool FULL, int DC, bool GAP, bool PE>
 __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   extern __shared__ double lds2[];
   const int c = blockIdx.x;
@@ -789,33 +809,39 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   const long pbase = (long)c * a.pcs;
   const long nstride = a.pns * (long)D;
   const float* __restrict__ Ab = a.A + pbase * D;
-  const float* __restrict__ wb = a.w + pbase;
+  const float* __restrict__ wb = a.w + pbase * a.wmul;
+  const float* __restrict__ wb2 = PE ? a.w2 + pbase * a.wmul : nullptr;
   float* msg = a.msg + (long)c * n * D;
   float* w_lds = reinterpret_cast<float*>(lds2);
   float* u_lds = w_lds + n;
+  float* w2_lds = u_lds + n;  // PE only
   const int strideB = (int)(nstride * 4);
   const Rsrc rA = chain_rsrc(Ab, (long)(n - 1) * nstride + D);
   const Rsrc rM = chain_rsrc(msg, (long)n * D);
   bool valid[LPL];
 #pragma unroll
   for (int j = 0; j < LPL; ++j) valid[j] = FULL || lane * LPL + j < D;
-  for (int i = threadIdx.x; i < n - 1; i += 128) w_lds[i] = wb[(long)i * a.pns];
+  const long wstep = a.pns * a.wmul;
+  for (int i = threadIdx.x; i < n - 1; i += 128) {
+    w_lds[i] = wb[(long)i * wstep];
+    if constexpr (PE) w2_lds[i] = wb2[(long)i * wstep];
+  }
   __syncthreads();
   const int mid = n / 2;
 
   Vec<LPL> X;  // wave 0: F_i, wave 1: Bh_i
-  auto fwd = [&](const Vec<LPL>& an, float w) {  // F_i = A_i + minconv(F_{i-1}; w_{i-1}) - min F_{i-1}
+  auto fwd = [&](const Vec<LPL>& an, float w, float w2) {  // F_i = A_i + minconv(F_{i-1}; w_{i-1}) - min F_{i-1}
     const float f = wave_min(local_min(X));
-    const Vec<LPL> m = minconv<LPL, DC>(X, f, w, L1v, L2v, a.delta, cc);
+    const Vec<LPL> m = minconv<LPL, DC, PE>(X, f, w, w2, L1v, L2v, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? an.v[k] + m.v[k] - f : kInfF;
   };
-  auto bwd = [&](int i, const Vec<LPL>& an1, float w) {  // Bh_i from Bh_{i+1}, A_{i+1} and w_i
+  auto bwd = [&](int i, const Vec<LPL>& an1, float w, float w2) {  // Bh_i from Bh_{i+1}, A_{i+1} and w_i
     Vec<LPL> U;
 #pragma unroll
     for (int k = 0; k < LPL; ++k) U.v[k] = (valid[k] ? an1.v[k] : kInfF) + X.v[k];
     const float u = wave_min(local_min(U));
-    const Vec<LPL> m = minconv<LPL, DC>(U, u, w, L1v, L2v, a.delta, cc);
+    const Vec<LPL> m = minconv<LPL, DC, PE>(U, u, w, w2, L1v, L2v, a.delta, cc);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = valid[k] ? m.v[k] - u : 0.f;
     u_lds[i] = u;  // every lane writes the same value
@@ -854,18 +880,18 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   if (wv == 0) {
     X = ld_node<LPL, FULL>(Ab, lane, D, kInfF);  // F_0 = A_0
     if (mid > 0) st_node<LPL, FULL>(msg, lane, D, X);
-    ring_pass<LPL, FULL, P, false>(rA, strideB, rM, w_lds, n, D, lane, 1, mid - 1, 1, 0, -1,
-                                   [&](int i, const Vec<LPL>& an, const Vec<LPL>&, float w) {
-                                     fwd(an, w);
+    ring_pass<LPL, FULL, P, false, PE>(rA, strideB, rM, w_lds, w2_lds, n, D, lane, 1, mid - 1, 1, 0, -1,
+                                       [&](int i, const Vec<LPL>& an, const Vec<LPL>&, float w, float w2) {
+                                         fwd(an, w, w2);
                                      st_buf<LPL, FULL>(rM, i * D * 4, lane, D, X);
                                    });
   } else {
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = 0.f;  // Bh_{n-1} = 0
     st_node<LPL, FULL>(msg + (long)(n - 1) * D, lane, D, X);
-    ring_pass<LPL, FULL, P, false>(rA, strideB, rM, w_lds, n, D, lane, n - 2, n - 1 - mid, -1, 1, 0,
-                                   [&](int i, const Vec<LPL>& an1, const Vec<LPL>&, float w) {
-                                     bwd(i, an1, w);
+    ring_pass<LPL, FULL, P, false, PE>(rA, strideB, rM, w_lds, w2_lds, n, D, lane, n - 2, n - 1 - mid, -1, 1, 0,
+                                       [&](int i, const Vec<LPL>& an1, const Vec<LPL>&, float w, float w2) {
+                                         bwd(i, an1, w, w2);
                                      st_buf<LPL, FULL>(rM, i * D * 4, lane, D, X);
                                    });
   }
@@ -877,25 +903,25 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   // first node of the half is handled before the ring, the last after it.
   Vec<LPL> Fp, Bp;  // min-marginal terms of the node whose decision is pending
   if (wv == 0) {     // nodes mid .. n-1, forward messages F
-    if (mid > 0) fwd(ld_buf<LPL, FULL>(rA, mid * strideB, lane, D, kInfF), w_lds[mid - 1]);
+    if (mid > 0) fwd(ld_buf<LPL, FULL>(rA, mid * strideB, lane, D, kInfF), w_lds[mid - 1], PE ? w2_lds[mid - 1] : 0.f);
     Fp = X;
     Bp = ld_buf<LPL, FULL>(rM, mid * D * 4, lane, D, 0.f);
-    ring_pass<LPL, FULL, P, true>(rA, strideB, rM, w_lds, n, D, lane, mid + 1, n - mid - 1, 1, 0, -1,
-                                  [&](int i, const Vec<LPL>& an, const Vec<LPL>& bh, float w) {
-                                    decide(i - 1, Fp, Bp);
-                                    fwd(an, w);
+    ring_pass<LPL, FULL, P, true, PE>(rA, strideB, rM, w_lds, w2_lds, n, D, lane, mid + 1, n - mid - 1, 1, 0, -1,
+                                      [&](int i, const Vec<LPL>& an, const Vec<LPL>& bh, float w, float w2) {
+                                        decide(i - 1, Fp, Bp);
+                                        fwd(an, w, w2);
                                     Fp = X;
                                     Bp = bh;
                                   });
     decide(n - 1, Fp, Bp);
   } else if (mid > 0) {  // nodes mid-1 .. 0, backward messages Bh
-    bwd(mid - 1, ld_buf<LPL, FULL>(rA, mid * strideB, lane, D, kInfF), w_lds[mid - 1]);
+    bwd(mid - 1, ld_buf<LPL, FULL>(rA, mid * strideB, lane, D, kInfF), w_lds[mid - 1], PE ? w2_lds[mid - 1] : 0.f);
     Fp = ld_buf<LPL, FULL>(rM, (mid - 1) * D * 4, lane, D, 0.f);
     Bp = X;
-    ring_pass<LPL, FULL, P, true>(rA, strideB, rM, w_lds, n, D, lane, mid - 2, mid - 1, -1, 1, 0,
-                                  [&](int i, const Vec<LPL>& an1, const Vec<LPL>& F, float w) {
-                                    decide(i + 1, Fp, Bp);
-                                    bwd(i, an1, w);
+    ring_pass<LPL, FULL, P, true, PE>(rA, strideB, rM, w_lds, w2_lds, n, D, lane, mid - 2, mid - 1, -1, 1, 0,
+                                      [&](int i, const Vec<LPL>& an1, const Vec<LPL>& F, float w, float w2) {
+                                        decide(i + 1, Fp, Bp);
+                                        bwd(i, an1, w, w2);
                                     Fp = F;
                                     Bp = X;
                                   });
@@ -912,42 +938,46 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   }
 }
 
-template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK = false>
+template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK, bool PE>
 static void launch_sweep(ChainArgs& a, size_t lds, hipStream_t st) {
   if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
-    (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK>,
+    (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK, PE>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK><<<a.n_chains, 64, lds, st>>>(a);
+  k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK, PE><<<a.n_chains, 64, lds, st>>>(a);
 }
 
-template <int LPL, int MODE, int P, bool FULL, int DC>
+template <int LPL, int MODE, int P, bool FULL, int DC, bool PE>
 static void launch_one(ChainArgs& a, size_t lds, bool seg, hipStream_t st) {
   if constexpr (MODE == kRowsDecode) {
-    const size_t lds2 = 2 * sizeof(float) * (size_t)a.n;
+    const size_t lds2 = (PE ? 3 : 2) * sizeof(float) * (size_t)a.n;
     if (a.gap)
-      k_decode2<LPL, P, FULL, DC, true><<<a.n_chains, 128, lds2, st>>>(a);
+      k_decode2<LPL, P, FULL, DC, true, PE><<<a.n_chains, 128, lds2, st>>>(a);
     else
-      k_decode2<LPL, P, FULL, DC, false><<<a.n_chains, 128, lds2, st>>>(a);
+      k_decode2<LPL, P, FULL, DC, false, PE><<<a.n_chains, 128, lds2, st>>>(a);
   } else if constexpr (MODE == kColsMinorant) {
     if (seg)
-      launch_sweep<LPL, MODE, P, FULL, DC, true>(a, lds, st);
+      launch_sweep<LPL, MODE, P, FULL, DC, true, false, PE>(a, lds, st);
     else
-      launch_sweep<LPL, MODE, P, FULL, DC, false>(a, lds, st);
+      launch_sweep<LPL, MODE, P, FULL, DC, false, false, PE>(a, lds, st);
   } else if (a.nan_flags) {
-    launch_sweep<LPL, MODE, P, FULL, DC, false, true>(a, lds, st);
+    launch_sweep<LPL, MODE, P, FULL, DC, false, true, PE>(a, lds, st);
   } else {
-    launch_sweep<LPL, MODE, P, FULL, DC, false>(a, lds, st);
+    launch_sweep<LPL, MODE, P, FULL, DC, false, false, PE>(a, lds, st);
   }
 }
 
 template <int MODE, int LPL, int P>
 static void launch_lpl(ChainArgs& a, size_t lds, bool seg, hipStream_t st) {
-  if (a.D == 64 * LPL && a.delta == 2)
-    launch_one<LPL, MODE, P, true, 2>(a, lds, seg, st);
-  else if (a.D == 64 * LPL)
-    launch_one<LPL, MODE, P, true, 0>(a, lds, seg, st);
-  else
-    launch_one<LPL, MODE, P, false, 0>(a, lds, seg, st);
+  if (a.w2) {  // per-edge penalties
+    if (a.D == 64 * LPL && a.delta == 2) launch_one<LPL, MODE, P, true, 2, true>(a, lds, seg, st);
+    else launch_one<LPL, MODE, P, false, 0, true>(a, lds, seg, st);
+  } else if (a.D == 64 * LPL && a.delta == 2) {
+    launch_one<LPL, MODE, P, true, 2, false>(a, lds, seg, st);
+  } else if (a.D == 64 * LPL) {
+    launch_one<LPL, MODE, P, true, 0, false>(a, lds, seg, st);
+  } else {
+    launch_one<LPL, MODE, P, false, 0, false>(a, lds, seg, st);
+  }
 }
 
 // Prefetch depth (nodes in flight per stream) by labels per lane; the row
@@ -962,7 +992,7 @@ static void launch_lpl(ChainArgs& a, size_t lds, bool seg, hipStream_t st) {
 
 template <int MODE>
 static int launch_mode(ChainArgs& a, bool seg, hipStream_t st) {
-  const size_t lds = (sizeof(double) + 2 * sizeof(float)) * (size_t)a.n;
+  const size_t lds = (sizeof(double) + (a.w2 ? 3 : 2) * sizeof(float)) * (size_t)a.n;
   constexpr int P2 = MODE == kColsMinorant ? CCS_CRF_P_COLS : CCS_CRF_P_ROWS;
   if (a.D <= 64) launch_lpl<MODE, 1, P2>(a, lds, seg, st);
   else if (a.D <= 128) launch_lpl<MODE, 2, P2>(a, lds, seg, st);

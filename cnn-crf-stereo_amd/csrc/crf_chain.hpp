@@ -14,7 +14,9 @@ struct ChainArgs {
   const float* cost;   // C (column mode)
   float* A;            // row unaries, updated in place
   const float* A_in;   // where a row sweep reads the unaries (A, or cost for the first sweep)
-  const float* w;      // edge weight of the edge (node i, node i+1), indexed by pixel
+  const float* w;      // edge weight of the edge (node i, node i+1), indexed by pixel (P1 with w2)
+  const float* w2;     // per-edge penalties: P2 of the edge (then w holds P1); nullptr: w L1, w L2
+  int wmul = 1;        // elements per pixel in the arrays of w / w2 (2: interleaved x, y)
   float* msg;          // backward messages, [chain][node][D]
   int* labels;         // decode output, indexed by pixel
   float* gap;          // decode: second best - best min-marginal (optional)

@@ -402,6 +402,26 @@ Tensor* SlackProp::forward(std::vector<Tensor*>& d_inputs, Handle& handle) {
   return m_output;
 }
 
+float SlackProp::forward(Tensor* costVolume, Tensor* pwVolume, Tensor* output, int max_iter, Handle* handle) {
+  if (!costVolume || !pwVolume || !output) arg_fail("SlackProp::forward: NULL tensor");
+  const TensorDesc& c = costVolume->desc();
+  const TensorDesc& p = pwVolume->desc();
+  const TensorDesc& o = output->desc();
+  // attention! NHWC cost volume only, as in the reference (slackprop.cpp:70)
+  if (c.layout != Layout::NHWC || c.n != 1 || c.cstride() != c.c || c.precision != Precision::FP32)
+    arg_fail("SlackProp::forward: the cost volume must be NHWC 1 x D x H x W float32");
+  if (p.layout != Layout::NCHW || p.n != 1 || p.h != c.h || p.w != c.w || p.precision != Precision::FP32)
+    arg_fail("SlackProp::forward: the pairwise volume must be NCHW 1 x 2 levels x H x W float32");
+  if (o.n != 1 || o.c != 1 || o.h != c.h || o.w != c.w || o.precision != Precision::FP32)
+    arg_fail("SlackProp::forward: the output must be 1 x 1 x H x W float32");
+  Handle& h = handle ? *handle : default_handle();
+  float lb = 0.f;
+  check(ccs_slackprop_forward_pw(h.get(), costVolume->data(), pwVolume->data(), output->data(), 1, c.c, c.h, c.w, p.c,
+                                 max_iter, &lb),
+        "SlackProp::forward");
+  return lb;
+}
+
 float SlackProp::forward(float* costVolume, float* wx, float* wy, float* output, int in, int ic, int ih, int iw,
                          float P1, float P2, int max_iter, Handle* handle) {
   Handle& h = handle ? *handle : default_handle();

@@ -237,6 +237,41 @@ def slackprop_forward_volume_out(h, cost, wx, wy, P1, P2, max_iter, delta):
     return out, lb.value
 
 
+def pairwise_penalties(h, pw):
+    """The reference's re-layout of a pairwise volume (2 levels x H x W) into the
+    per-edge penalties (levels x H x W x 2) and the per-block validity flags."""
+    C2, H, W = pw.shape
+    pen = torch.empty((C2 // 2, H, W, 2), dtype=torch.float32, device="cuda")
+    bad = torch.empty(((H * W + 255) // 256,), dtype=torch.int32, device="cuda")
+    call("ccs_pairwise_penalties", h.h, ptr(pw), C2, H, W, ptr(pen), ptr(bad))
+    return pen, bad
+
+
+def slackprop_pw(h, cost, pen, delta, iters, want_unaries=False):
+    """ccs_slackprop with per-edge penalties pen (levels x H x W x 2)."""
+    H, W, D = cost.shape
+    labels = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    sums = torch.zeros(4 * iters + 1, dtype=torch.float64, device="cuda")
+    lb = torch.zeros(1, dtype=torch.float64, device="cuda")
+    un = torch.empty_like(cost) if want_unaries else None
+    call("ccs_slackprop_pw", h.h, ptr(cost), ptr(pen), pen.shape[0], H, W, D, int(delta), int(iters), ptr(labels),
+         ptr(un), ptr(sums), ptr(lb))
+    out = dict(labels=labels, sums=sums, lb=lb)
+    if want_unaries:
+        out["unaries"] = un
+    return out
+
+
+def slackprop_forward_pw(h, cost, pw, max_iter=5):
+    """The reference's SlackProp::forward(costVolume, pwVolume, output, max_iter): labels as float and the bound."""
+    H, W, D = cost.shape
+    out = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    lb = ctypes.c_float()
+    call("ccs_slackprop_forward_pw", h.h, ptr(cost), ptr(pw), ptr(out), 1, D, H, W, pw.shape[0], int(max_iter),
+         ctypes.byref(lb))
+    return out, lb.value
+
+
 def slackprop_status(h):
     """Raises CCSError (status ERR_NAN) if the last CRF run on h read a NaN cost."""
     call("ccs_slackprop_status", h.h)

@@ -228,6 +228,39 @@ int ccs_slackprop(ccs_handle_t h, const float* cost, const float* wx, const floa
                   int W, int D, float L1, float L2, int delta, int iters, int* labels,
                   float* unaries_out, double* sums, double* lower_bound);
 
+/* SlackProp::forward, per-edge ("full") interface (stereo/operators/slackprop.h:40,
+ * slackprop.cpp:71-108): cost_volume ih x iw x ic (ic labels), pw_volume the
+ * pairwise volume 1 x pw_channels x ih x iw (NCHW planes in the reference's
+ * channel order P1x, P1y, P2x, P2y; x: the edge (x,y)-(x+1,y), y: (x,y)-(x,y+1)),
+ * output ih x iw labels written as float; *lower_bound the dual bound.
+ * pw_channels 4 gives rho_e(d) = 0, P1_e for |d| = 1, P2_e for |d| >= 2 (delta
+ * = 2); pw_channels 2 the Potts form rho_e(d) = P_e for d != 0.  The penalties
+ * must satisfy 0 <= P1 <= P2 on every edge (CCS_ERR_ARG otherwise).  The
+ * reference passes the volume to an un-vendored solver whose use of the levels
+ * is not visible; this is the restated meaning.  Synchronises; CCS_ERR_NAN as
+ * ccs_slackprop_forward. */
+int ccs_slackprop_forward_pw(ccs_handle_t h, const float* cost_volume, const float* pw_volume,
+                             float* output, int in, int ic, int ih, int iw, int pw_channels,
+                             int max_iter, float* lower_bound);
+
+/* The re-layout of the pairwise volume that the reference makes before its
+ * solver (stereo/operators/slackprop.cpp:88-92): pen (levels x H x W x 2,
+ * levels = pw_channels / 2) = pw (pw_channels x H x W) split into (2, levels)
+ * with x/y fastest, levels flipped, x/y moved last:
+ * pen[l][y][x][c] = pw[c + 2 (levels-1-l)][y][x].  bad (ceil(H W / 256) ints)
+ * flags the pixel blocks with an edge that breaks 0 <= P1 <= P2.  Stream-ordered. */
+int ccs_pairwise_penalties(ccs_handle_t h, const float* pw, int pw_channels, int H, int W, float* pen,
+                           int* bad);
+
+/* ccs_slackprop with per-edge penalties pen (levels x H x W x 2, as
+ * ccs_pairwise_penalties leaves them: P1 = pen[levels-1], P2 = pen[0]; the x
+ * components price the row edges, the y components the column edges) instead
+ * of edge weights times L1, L2.  levels 1 or 2; delta as in ccs_slackprop.
+ * Stream-ordered. */
+int ccs_slackprop_pw(ccs_handle_t h, const float* cost, const float* pen, int levels, int H, int W,
+                     int D, int delta, int iters, int* labels, float* unaries_out, double* sums,
+                     double* lower_bound);
+
 /* Waits for the handle's stream and returns CCS_ERR_NAN if the first row sweep
  * of the last ccs_slackprop run on this handle (or the last direction-3
  * ccs_slackprop_sweep) read a NaN cost, CCS_OK otherwise (also when no such

@@ -307,6 +307,11 @@ class SlackProp : public Operator {
   void setIterations(int value) { m_iters = value; }
   float lowerBound() const { return m_lowerBound; }
 
+  // The per-edge ("full") interface (slackprop.h:40): costVolume 1 x D x H x W
+  // (NHWC), pwVolume 1 x 2 levels x H x W (NCHW planes P1x, P1y, P2x, P2y, or
+  // P x, P y for the Potts form), output 1 x 1 x H x W labels as float.
+  static float forward(Tensor* costVolume, Tensor* pwVolume, Tensor* output, int max_iter = 5,
+                       Handle* handle = nullptr);
   // The reference's raw-pointer statics (slackprop.h:43-47).
   static float forward(float* costVolume, float* wx, float* wy, float* output, int in, int ic, int ih, int iw,
                        float P1, float P2, int max_iter = 5, Handle* handle = nullptr);

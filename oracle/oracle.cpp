@@ -380,18 +380,22 @@ extern "C" double or_energy_masked(const double* C, const double* wx, const doub
 
 namespace {
 
-// out(s) = min_t U(t) + w rho(s - t), exact for 0 <= L1 <= L2 and w >= 0:
-// min(U(s), min_{0<|s-t|<delta} U(t) + w L1, min_t U(t) + w L2).
-void minconv(const double* U, int D, double w, double L1, double L2, int delta, double* out) {
+// The chains below take each edge's penalties P1 (0 < |d| < delta) and P2
+// (|d| >= delta): w L1 and w L2 for the edge-weight form, per-edge values for
+// the pairwise-volume form.
+//
+// out(s) = min_t U(t) + rho_e(s - t), exact for 0 <= P1 <= P2:
+// min(U(s), min_{0<|s-t|<delta} U(t) + P1, min_t U(t) + P2).
+void minconv(const double* U, int D, double P1, double P2, int delta, double* out) {
   double g = kInf;
   for (int t = 0; t < D; ++t) g = std::min(g, U[t]);
-  const double far = g + w * L2;
+  const double far = g + P2;
   for (int s = 0; s < D; ++s) {
     double best = U[s];
     double near = kInf;
     for (int t = std::max(0, s - delta + 1); t <= std::min(D - 1, s + delta - 1); ++t)
       if (t != s) near = std::min(near, U[t]);
-    best = std::min(best, near + w * L1);
+    best = std::min(best, near + P1);
     best = std::min(best, far);
     out[s] = best;
   }
@@ -407,8 +411,8 @@ double vmin(const double* v, int D) {
 // Bh[n-1] = 0; U = theta[i+1] + Bh[i+1]; u_i = min U; Bh[i] = minconv(U) - u_i.
 // The true backward message is B_i = Bh[i] + b_i with b_{n-1} = 0,
 // b_i = b_{i+1} + u_i; b (n entries) receives these offsets.
-void backward_pass(const double* theta, const double* w, int n, int D, double L1, double L2,
-                   int delta, double* Bh, double* b) {
+void backward_pass(const double* theta, const double* p1, const double* p2, int n, int D, int delta, double* Bh,
+                   double* b) {
   std::vector<double> U(D);
   for (int k = 0; k < D; ++k) Bh[(long)(n - 1) * D + k] = 0.0;
   b[n - 1] = 0.0;
@@ -416,21 +420,21 @@ void backward_pass(const double* theta, const double* w, int n, int D, double L1
     for (int k = 0; k < D; ++k) U[k] = theta[(long)(i + 1) * D + k] + Bh[(long)(i + 1) * D + k];
     const double u = vmin(U.data(), D);
     b[i] = b[i + 1] + u;
-    minconv(U.data(), D, w[i], L1, L2, delta, Bh + (long)i * D);
+    minconv(U.data(), D, p1[i], p2[i], delta, Bh + (long)i * D);
     for (int k = 0; k < D; ++k) Bh[(long)i * D + k] -= u;
   }
 }
 
-// excess(s) = min_t [w rho(s - t) - m(t)] + V(s) for the message m = minconv(V):
+// excess(s) = min_t [rho_e(s - t) - m(t)] + V(s) for the message m = minconv(V):
 // the part of V(s) that the message to the next node cannot see.  Because
-// m(t) >= min V >= max m - w L2, the far term never binds, so
-// excess(s) = V(s) - max(m(s), max_{0<|s-t|<delta} m(t) - w L1)  (>= 0, min 0).
-void excess(const double* V, const double* m, int D, double w, double L1, int delta, double* e) {
+// m(t) >= min V >= max m - P2, the far term never binds, so
+// excess(s) = V(s) - max(m(s), max_{0<|s-t|<delta} m(t) - P1)  (>= 0, min 0).
+void excess(const double* V, const double* m, int D, double P1, int delta, double* e) {
   for (int s = 0; s < D; ++s) {
     double near = -kInf;
     for (int t = std::max(0, s - delta + 1); t <= std::min(D - 1, s + delta - 1); ++t)
       if (t != s) near = std::max(near, m[t]);
-    e[s] = V[s] - std::max(m[s], near - w * L1);
+    e[s] = V[s] - std::max(m[s], near - P1);
   }
 }
 
@@ -442,12 +446,12 @@ void excess(const double* V, const double* m, int D, double w, double L1, int de
 // g is stored without its per-node constant kappa_i (g_i = g~_i + kappa_i); the
 // constants follow the scalar recurrence below and *K receives sum_i kappa_i,
 // which is exactly the chain minimum left behind when g~ is subtracted.
-double minorant_impl(const double* theta, const double* w, int n, int D, double L1, double L2,
-                     int delta, double* g, double* K, std::vector<double>& Bh) {
+double minorant_impl(const double* theta, const double* p1, const double* p2, int n, int D, int delta, double* g,
+                     double* K, std::vector<double>& Bh) {
   constexpr double gamma = 0.25;
   Bh.resize((long)n * D);
   std::vector<double> b(n);
-  backward_pass(theta, w, n, D, L1, L2, delta, Bh.data(), b.data());
+  backward_pass(theta, p1, p2, n, D, delta, Bh.data(), b.data());
   double m0 = kInf;
   for (int k = 0; k < D; ++k) m0 = std::min(m0, theta[k] + Bh[k]);
   const double Lambda = b[0] + m0;
@@ -461,8 +465,8 @@ double minorant_impl(const double* theta, const double* w, int n, int D, double 
     }
     kept += gamma * (r + b[i]);
     const double v = vmin(V.data(), D);
-    minconv(V.data(), D, w[i], L1, L2, delta, msg.data());
-    excess(V.data(), msg.data(), D, w[i], L1, delta, e.data());
+    minconv(V.data(), D, p1[i], p2[i], delta, msg.data());
+    excess(V.data(), msg.data(), D, p1[i], delta, e.data());
     for (int k = 0; k < D; ++k) {
       gi[k] += e[k];
       R[k] = theta[(long)(i + 1) * D + k] + msg[k] - v;
@@ -479,18 +483,18 @@ double minorant_impl(const double* theta, const double* w, int n, int D, double 
 // (lowest label among those within kTieEps of the minimum); gap = the best
 // value over the other labels minus the minimum (0 when the chosen label is
 // not the exact minimum).
-double decode_impl(const double* theta, const double* w, int n, int D, double L1, double L2,
-                   int delta, int* labels, double* gap, std::vector<double>& Bh) {
+double decode_impl(const double* theta, const double* p1, const double* p2, int n, int D, int delta, int* labels,
+                   double* gap, std::vector<double>& Bh) {
   Bh.resize((long)n * D);
   std::vector<double> b(n);
-  backward_pass(theta, w, n, D, L1, L2, delta, Bh.data(), b.data());
+  backward_pass(theta, p1, p2, n, D, delta, Bh.data(), b.data());
   double m0 = kInf;
   for (int k = 0; k < D; ++k) m0 = std::min(m0, theta[k] + Bh[k]);
   std::vector<double> F(theta, theta + D), msg(D);
   for (int i = 0; i < n; ++i) {
     if (i > 0) {
       const double f = vmin(F.data(), D);
-      minconv(F.data(), D, w[i - 1], L1, L2, delta, msg.data());
+      minconv(F.data(), D, p1[i - 1], p2[i - 1], delta, msg.data());
       for (int k = 0; k < D; ++k) F[k] = theta[(long)i * D + k] + msg[k] - f;
     }
     // Lowest label whose min-marginal is within kTieEps (relative) of the
@@ -509,6 +513,73 @@ double decode_impl(const double* theta, const double* w, int n, int D, double L1
     if (gap) gap[i] = second - best;
   }
   return b[0] + m0;
+}
+
+// Per-edge penalties of a chain of n nodes from edge weights: P1 = w L1, P2 = w L2.
+void weighted(const double* w, int n, double L1, double L2, std::vector<double>& p1, std::vector<double>& p2) {
+  p1.assign(n > 1 ? n - 1 : 0, 0.0);
+  p2.assign(p1.size(), 0.0);
+  for (int i = 0; i + 1 < n; ++i) {
+    p1[i] = w[i] * L1;
+    p2[i] = w[i] * L2;
+  }
+}
+
+// The restated SlackProp on per-edge penalties: rows use the horizontal
+// edges' (p1x, p2x)[y][x] (edge (x,y)-(x+1,y)), columns the vertical edges'
+// (p1y, p2y)[y][x] (edge (x,y)-(x,y+1)).  See or_slackprop.
+double slackprop_impl(const double* C, const double* p1x, const double* p2x, const double* p1y, const double* p2y,
+                      int H, int W, int D, int delta, int iters, int* labels, double* sweep_sums, double* kept_sums,
+                      double* gap, double* A_out) {
+  const long HW = (long)H * W;
+  std::vector<double> A(C, C + HW * D);
+  std::vector<double> sums(2 * (long)iters + 1, 0.0), kept(2 * (long)iters, 0.0);
+  int j = 0;
+  for (int it = 0; it < iters; ++it) {
+    double srow = 0.0, krow = 0.0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : srow, krow)
+    for (int y = 0; y < H; ++y) {
+      std::vector<double> g((long)W * D), Bh;
+      double* Ar = A.data() + (long)y * W * D;
+      double k = 0.0;
+      srow += minorant_impl(Ar, p1x + (long)y * W, p2x + (long)y * W, W, D, delta, g.data(), &k, Bh);
+      krow += k;
+      for (long i = 0; i < (long)W * D; ++i) Ar[i] -= g[i];
+    }
+    sums[j] = srow;
+    kept[j] = krow;
+    ++j;
+    double scol = 0.0, kcol = 0.0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : scol, kcol)
+    for (int x = 0; x < W; ++x) {
+      std::vector<double> th((long)H * D), g((long)H * D), c1(H), c2(H), Bh;
+      for (int y = 0; y < H; ++y) {
+        c1[y] = p1y[(long)y * W + x];
+        c2[y] = p2y[(long)y * W + x];
+        for (int k = 0; k < D; ++k) th[(long)y * D + k] = C[((long)y * W + x) * D + k] - A[((long)y * W + x) * D + k];
+      }
+      double k = 0.0;
+      scol += minorant_impl(th.data(), c1.data(), c2.data(), H, D, delta, g.data(), &k, Bh);
+      kcol += k;
+      for (int y = 0; y < H; ++y)
+        for (int k = 0; k < D; ++k) A[((long)y * W + x) * D + k] += g[(long)y * D + k];
+    }
+    sums[j] = scol;
+    kept[j] = kcol;
+    ++j;
+  }
+  double sdec = 0.0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : sdec)
+  for (int y = 0; y < H; ++y) {
+    std::vector<double> Bh;
+    sdec += decode_impl(A.data() + (long)y * W * D, p1x + (long)y * W, p2x + (long)y * W, W, D, delta,
+                        labels + (long)y * W, gap ? gap + (long)y * W : nullptr, Bh);
+  }
+  sums[j] = sdec;
+  if (sweep_sums) std::memcpy(sweep_sums, sums.data(), sizeof(double) * sums.size());
+  if (kept_sums && iters > 0) std::memcpy(kept_sums, kept.data(), sizeof(double) * kept.size());
+  if (A_out) std::memcpy(A_out, A.data(), sizeof(double) * HW * D);
+  return iters == 0 ? sdec : kept[j - 1] + sdec;
 }
 
 }  // namespace
@@ -548,8 +619,9 @@ extern "C" double or_chain_viterbi(const double* theta, const double* w, int n, 
 
 extern "C" double or_chain_minorant(const double* theta, const double* w, int n, int D, double L1,
                                     double L2, int delta, double* g, double* kept) {
-  std::vector<double> Bh;
-  return minorant_impl(theta, w, n, D, L1, L2, delta, g, kept, Bh);
+  std::vector<double> Bh, p1, p2;
+  weighted(w, n, L1, L2, p1, p2);
+  return minorant_impl(theta, p1.data(), p2.data(), n, D, delta, g, kept, Bh);
 }
 
 // Restated SlackProp.  State: row unaries A (the columns hold C - A).
@@ -569,53 +641,52 @@ extern "C" double or_slackprop(const double* C, const double* wx, const double* 
                                int D, double L1, double L2, int delta, int iters, int* labels,
                                double* sweep_sums, double* kept_sums, double* gap, double* A_out) {
   const long HW = (long)H * W;
-  std::vector<double> A(C, C + HW * D);
-  std::vector<double> sums(2 * (long)iters + 1, 0.0), kept(2 * (long)iters, 0.0);
-  int j = 0;
-  for (int it = 0; it < iters; ++it) {
-    double srow = 0.0, krow = 0.0;
-#pragma omp parallel for schedule(dynamic) reduction(+ : srow, krow)
-    for (int y = 0; y < H; ++y) {
-      std::vector<double> g((long)W * D), Bh;
-      double* Ar = A.data() + (long)y * W * D;
-      double k = 0.0;
-      srow += minorant_impl(Ar, wx + (long)y * W, W, D, L1, L2, delta, g.data(), &k, Bh);
-      krow += k;
-      for (long i = 0; i < (long)W * D; ++i) Ar[i] -= g[i];
-    }
-    sums[j] = srow;
-    kept[j] = krow;
-    ++j;
-    double scol = 0.0, kcol = 0.0;
-#pragma omp parallel for schedule(dynamic) reduction(+ : scol, kcol)
+  std::vector<double> p1x(HW), p2x(HW), p1y(HW), p2y(HW);
+  for (long p = 0; p < HW; ++p) {
+    p1x[p] = wx[p] * L1;
+    p2x[p] = wx[p] * L2;
+    p1y[p] = wy[p] * L1;
+    p2y[p] = wy[p] * L2;
+  }
+  return slackprop_impl(C, p1x.data(), p2x.data(), p1y.data(), p2y.data(), H, W, D, delta, iters, labels, sweep_sums,
+                        kept_sums, gap, A_out);
+}
+
+// The pairwise-volume form of SlackProp::forward (stereo/operators/slackprop.cpp:71-108):
+// per-edge penalties instead of weighted L1, L2.  The state, schedule, bound
+// and decode are those of or_slackprop.
+extern "C" double or_slackprop_pe(const double* C, const double* p1x, const double* p2x, const double* p1y,
+                                  const double* p2y, int H, int W, int D, int delta, int iters, int* labels,
+                                  double* sweep_sums, double* kept_sums, double* gap, double* A_out) {
+  return slackprop_impl(C, p1x, p2x, p1y, p2y, H, W, D, delta, iters, labels, sweep_sums, kept_sums, gap, A_out);
+}
+
+// E(x) with per-edge penalties: rho_e(d) = 0, P1 (0 < |d| < delta), P2 (|d| >= delta).
+extern "C" double or_energy_pe(const double* C, const double* p1x, const double* p2x, const double* p1y,
+                               const double* p2y, int H, int W, int D, const int* labels, int delta) {
+  auto r = [delta](int d, double P1, double P2) {
+    const int a = d < 0 ? -d : d;
+    return a == 0 ? 0.0 : (a < delta ? P1 : P2);
+  };
+  double e = 0.0;
+  for (int y = 0; y < H; ++y)
     for (int x = 0; x < W; ++x) {
-      std::vector<double> th((long)H * D), g((long)H * D), wc(H), Bh;
-      for (int y = 0; y < H; ++y) {
-        wc[y] = wy[(long)y * W + x];
-        for (int k = 0; k < D; ++k) th[(long)y * D + k] = C[((long)y * W + x) * D + k] - A[((long)y * W + x) * D + k];
-      }
-      double k = 0.0;
-      scol += minorant_impl(th.data(), wc.data(), H, D, L1, L2, delta, g.data(), &k, Bh);
-      kcol += k;
-      for (int y = 0; y < H; ++y)
-        for (int k = 0; k < D; ++k) A[((long)y * W + x) * D + k] += g[(long)y * D + k];
+      const long p = (long)y * W + x;
+      const int l = labels[p];
+      e += C[p * D + l];
+      if (x + 1 < W) e += r(l - labels[p + 1], p1x[p], p2x[p]);
+      if (y + 1 < H) e += r(l - labels[p + W], p1y[p], p2y[p]);
     }
-    sums[j] = scol;
-    kept[j] = kcol;
-    ++j;
-  }
-  double sdec = 0.0;
-#pragma omp parallel for schedule(dynamic) reduction(+ : sdec)
-  for (int y = 0; y < H; ++y) {
-    std::vector<double> Bh;
-    sdec += decode_impl(A.data() + (long)y * W * D, wx + (long)y * W, W, D, L1, L2, delta,
-                        labels + (long)y * W, gap ? gap + (long)y * W : nullptr, Bh);
-  }
-  sums[j] = sdec;
-  if (sweep_sums) std::memcpy(sweep_sums, sums.data(), sizeof(double) * sums.size());
-  if (kept_sums && iters > 0) std::memcpy(kept_sums, kept.data(), sizeof(double) * kept.size());
-  if (A_out) std::memcpy(A_out, A.data(), sizeof(double) * HW * D);
-  return iters == 0 ? sdec : kept[j - 1] + sdec;
+  return e;
+}
+
+// The reference's re-layout of the pairwise volume (stereo/operators/slackprop.cpp:88-92):
+// pen[l][y][x][c] = pw[c + 2 (levels - 1 - l)][y][x] for pw of 2 levels planes.
+extern "C" void or_pw_relayout(const double* pw, int levels, int H, int W, double* pen) {
+  const long HW = (long)H * W;
+  for (int l = 0; l < levels; ++l)
+    for (long p = 0; p < HW; ++p)
+      for (int c = 0; c < 2; ++c) pen[((long)l * HW + p) * 2 + c] = pw[(long)(c + 2 * (levels - 1 - l)) * HW + p];
 }
 
 extern "C" void or_ssvm_grad(const int* xhat, const int* xstar, const double* wx, const double* wy,

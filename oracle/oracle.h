@@ -120,6 +120,22 @@ double or_slackprop(const double* C, const double* wx, const double* wy, int H, 
                     double L1, double L2, int delta, int iters, int* labels,
                     double* sweep_sums, double* kept_sums, double* gap, double* A_out);
 
+/* or_slackprop with per-edge penalties (the pairwise-volume form of SlackProp::forward,
+ * stereo/operators/slackprop.cpp:71-108): the horizontal edge (x,y)-(x+1,y) costs
+ * p1x[y][x] for 0 < |d| < delta and p2x[y][x] for |d| >= delta, the vertical edge
+ * (x,y)-(x,y+1) p1y / p2y.  Same schedule, bound and decode as or_slackprop. */
+double or_slackprop_pe(const double* C, const double* p1x, const double* p2x, const double* p1y,
+                       const double* p2y, int H, int W, int D, int delta, int iters, int* labels,
+                       double* sweep_sums, double* kept_sums, double* gap, double* A_out);
+
+/* The CRF energy with per-edge penalties (the energy or_slackprop_pe bounds). */
+double or_energy_pe(const double* C, const double* p1x, const double* p2x, const double* p1y,
+                    const double* p2y, int H, int W, int D, const int* labels, int delta);
+
+/* The pairwise volume's re-layout (stereo/operators/slackprop.cpp:88-92):
+ * pen[l][y][x][c] = pw[c + 2 (levels - 1 - l)][y][x]. */
+void or_pw_relayout(const double* pw, int levels, int H, int W, double* pen);
+
 /* Structured-SVM subgradient of  E(x*) - E(xhat)  w.r.t. unaries, L1, L2 and edge weights.
  * dC (H x W x D) gets +1 at x*, -1 at xhat; dL[0], dL[1] get dE/dL1, dE/dL2 differences;
  * dwx, dwy (H x W, optional) get rho differences.  Pixels with x* outside [0, D) are

@@ -55,6 +55,10 @@ def lib():
             "or_chain_minorant": (c_double, [_D, _D, c_int, c_int, c_double, c_double, c_int, _D, _D]),
             "or_slackprop": (c_double, [_D, _D, _D, c_int, c_int, c_int, c_double, c_double, c_int,
                                         c_int, _I, _D, _D, _D, _D]),
+            "or_slackprop_pe": (c_double, [_D, _D, _D, _D, _D, c_int, c_int, c_int, c_int, c_int, _I, _D, _D, _D,
+                                           _D]),
+            "or_energy_pe": (c_double, [_D, _D, _D, _D, _D, c_int, c_int, c_int, _I, c_int]),
+            "or_pw_relayout": (None, [_D, c_int, c_int, c_int, _D]),
             "or_ssvm_grad": (None, [_I, _I, _D, _D, c_int, c_int, c_int, c_double, c_double, c_int,
                                     _D, _D, _D, _D]),
         }
@@ -269,6 +273,48 @@ def slackprop(C, wx, wy, L1, L2, delta, iters, want_A=False):
     if want_A:
         out["A"] = A
     return out
+
+
+def slackprop_pe(C, p1x, p2x, p1y, p2y, delta, iters, want_A=False):
+    """SlackProp with per-edge penalties; returns dict(labels, lb, sums, kept, gap[, A])."""
+    C = _d(C)
+    p1x, p2x, p1y, p2y = _d(p1x), _d(p2x), _d(p1y), _d(p2y)
+    H, W, D = C.shape
+    lab = np.empty((H, W), dtype=np.int32)
+    sums = np.empty(2 * iters + 1)
+    kept = np.empty(max(2 * iters, 1))
+    gap = np.empty((H, W))
+    A = np.empty_like(C) if want_A else None
+    lb = lib().or_slackprop_pe(_pd(C), _pd(p1x), _pd(p2x), _pd(p1y), _pd(p2y), H, W, D, int(delta), iters,
+                               _pi(lab), _pd(sums), _pd(kept), _pd(gap), _pd(A))
+    out = dict(labels=lab, lb=lb, sums=sums, kept=kept[:2 * iters], gap=gap)
+    if want_A:
+        out["A"] = A
+    return out
+
+
+def energy_pe(C, p1x, p2x, p1y, p2y, labels, delta):
+    C = _d(C)
+    H, W, D = C.shape
+    lab = _i(labels)
+    return lib().or_energy_pe(_pd(C), _pd(_d(p1x)), _pd(_d(p2x)), _pd(_d(p1y)), _pd(_d(p2y)), H, W, D, _pi(lab),
+                              int(delta))
+
+
+def pw_relayout(pw):
+    """pw: (2 levels, H, W) -> pen (levels, H, W, 2), the reference's re-layout."""
+    pw = _d(pw)
+    C2, H, W = pw.shape
+    pen = np.empty((C2 // 2, H, W, 2))
+    lib().or_pw_relayout(_pd(pw), C2 // 2, H, W, _pd(pen))
+    return pen
+
+
+def pw_penalties(pw):
+    """Per-edge (p1x, p2x, p1y, p2y) of a pairwise volume: P1 = pen[levels-1], P2 = pen[0]."""
+    pen = pw_relayout(pw)
+    L = pen.shape[0]
+    return pen[L - 1, :, :, 0], pen[0, :, :, 0], pen[L - 1, :, :, 1], pen[0, :, :, 1]
 
 
 def ssvm_grad(xhat, xstar, wx, wy, D, L1, L2, delta):

@@ -481,3 +481,78 @@ def test_ssvm_grad_with_unlabelled_pixels():
         e[y, x] = 1.0
         np.testing.assert_allclose(dwx[y, x], diff(L1, L2, wx + e, wy) - base, atol=1e-12)
         np.testing.assert_allclose(dwy[y, x], diff(L1, L2, wx, wy + e) - base, atol=1e-12)
+
+
+# ---------------------------------------------------------------- per-edge CRF
+
+def test_pw_relayout_matches_reshape_flip_permute():
+    """The pairwise volume's re-layout as the reference writes it with
+    ndarray (stereo/operators/slackprop.cpp:88-92): channels split into
+    (2, levels) with x/y fastest, levels flipped, x/y moved last."""
+    rng = np.random.default_rng(20)
+    for levels in (1, 2, 3):
+        pw = rng.normal(size=(2 * levels, 4, 5))
+        want = np.transpose(pw.reshape(levels, 2, 4, 5)[::-1], (0, 2, 3, 1))
+        np.testing.assert_array_equal(O.pw_relayout(pw), want)
+    # the reference's channel order P1x, P1y, P2x, P2y: P1 = pen[levels - 1], P2 = pen[0]
+    pw = np.stack([np.full((2, 3), v) for v in (1.0, 2.0, 3.0, 4.0)])
+    p1x, p2x, p1y, p2y = O.pw_penalties(pw)
+    assert (p1x == 1).all() and (p1y == 2).all() and (p2x == 3).all() and (p2y == 4).all()
+
+
+def test_slackprop_pe_with_weighted_penalties_is_slackprop():
+    """Per-edge penalties P1 = w L1, P2 = w L2 reproduce or_slackprop: labels and
+    unaries bit for bit, the float64 sums to their summation order (OpenMP
+    reductions add the chains in a run-dependent order)."""
+    rng = np.random.default_rng(21)
+    H, W, D = 5, 7, 6
+    C = rng.normal(size=(H, W, D))
+    wx, wy = rng.uniform(0.2, 1, (H, W)), rng.uniform(0.2, 1, (H, W))
+    r = O.slackprop(C, wx, wy, 0.5, 2.0, 2, 3, want_A=True)
+    q = O.slackprop_pe(C, wx * 0.5, wx * 2.0, wy * 0.5, wy * 2.0, 2, 3, want_A=True)
+    assert abs(r["lb"] - q["lb"]) <= 1e-12 * abs(r["lb"])
+    np.testing.assert_array_equal(r["labels"], q["labels"])
+    np.testing.assert_array_equal(r["A"], q["A"])
+    np.testing.assert_allclose(r["sums"], q["sums"], rtol=1e-12)
+
+
+def _brute_min(C, p1x, p2x, p1y, p2y, delta):
+    H, W, D = C.shape
+    best = np.inf
+    for lab in itertools.product(range(D), repeat=H * W):
+        e = O.energy_pe(C, p1x, p2x, p1y, p2y, np.array(lab, dtype=np.int32).reshape(H, W), delta)
+        best = min(best, e)
+    return best
+
+
+def test_slackprop_pe_single_row_is_exact():
+    """A 1-row grid is one chain: the bound equals the exhaustive minimum of
+    the per-edge energy, and the decoded labelling attains it."""
+    rng = np.random.default_rng(22)
+    H, W, D = 1, 6, 3
+    C = rng.normal(size=(H, W, D))
+    p1x = rng.uniform(0, 1, (H, W))
+    p2x = p1x + rng.uniform(0, 2, (H, W))
+    zero = np.zeros((H, W))
+    r = O.slackprop_pe(C, p1x, p2x, zero, zero, 2, 2)
+    emin = _brute_min(C, p1x, p2x, zero, zero, 2)
+    assert abs(r["lb"] - emin) < 1e-10
+    assert abs(O.energy_pe(C, p1x, p2x, zero, zero, r["labels"], 2) - emin) < 1e-10
+
+
+def test_slackprop_pe_grid_brackets_the_minimum():
+    """3 x 3 pixels, D = 3, independent per-edge penalties (and the Potts
+    form P1 = P2): LB <= min E <= E(labels) by exhaustive search."""
+    rng = np.random.default_rng(23)
+    H, W, D = 3, 3, 3
+    C = rng.normal(size=(H, W, D))
+    for potts in (False, True):
+        p1x, p1y = rng.uniform(0, 1.5, (H, W)), rng.uniform(0, 1.5, (H, W))
+        p2x = p1x if potts else p1x + rng.uniform(0, 2, (H, W))
+        p2y = p1y if potts else p1y + rng.uniform(0, 2, (H, W))
+        r = O.slackprop_pe(C, p1x, p2x, p1y, p2y, 2, 6)
+        emin = _brute_min(C, p1x, p2x, p1y, p2y, 2)
+        e_lab = O.energy_pe(C, p1x, p2x, p1y, p2y, r["labels"], 2)
+        assert r["lb"] <= emin + 1e-10 <= e_lab + 2e-10
+        seq = O.dual_sequence(r["sums"], r["kept"])
+        assert all(b >= a - 1e-9 for a, b in zip(seq, seq[1:]))
