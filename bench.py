@@ -307,21 +307,11 @@ def bad_pixel_rate(labels, gt, D, border=0):
     return float(err.mean())
 
 
-def cpu_baseline(data, spec, params, pad, rows):
-    """The FP64 oracle of the same pipeline on a band of `rows` image rows."""
+def oracle_frame(O, left, right, spec, params, pad):
+    """The FP64 oracle of the whole pipeline on planar float64 images."""
     import numpy as np
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-
-    H = data["left"].shape[1]
-    rows = H if rows <= 0 or rows > H else rows
-    y0 = max(0, H // 2 - rows // 2)
-    band = slice(y0, y0 + rows)
-    left = data["left"][:, band, :].astype(np.float64)
-    right = data["right"][:, band, :].astype(np.float64)
     D = spec["D"]
-    t0 = time.perf_counter()
     feats = []
     for img in (left, right):
         x, _, _ = O.normalize(O.pad_symmetric(img, pad))
@@ -330,13 +320,96 @@ def cpu_baseline(data, spec, params, pad, rows):
         feats.append(x)
     S = O.correlation(feats[0], feats[1], -np.arange(D))
     cost = O.neg_softmax(S)
-    wx, wy = O.edge_weights(left[0], 0.7, 10.0, 1.0, 1.0)
+    gray = O.rgb2gray(left) if left.shape[0] == 3 else left[0]
+    wx, wy = O.edge_weights(gray, 0.7, 10.0, 1.0, 1.0)
     r = O.slackprop(cost, wx, wy, 0.5, 2.0, 2, spec["iters"])
-    dt = time.perf_counter() - t0
+    return feats, cost, wx, wy, r
+
+
+def cpu_baseline(data, spec, params, pad, rows, repeats=3, time_rows=0):
+    """The FP64 oracle of the same pipeline on a band of `rows` image rows
+    (run once: its outputs are the reference of the GPU error figures), then
+    timed as the median of `repeats` runs on a band of `time_rows` rows (the
+    first run is the warm-up)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    H = data["left"].shape[1]
+
+    def band(n):
+        n = H if n <= 0 or n > H else n
+        y0 = max(0, H // 2 - n // 2)
+        sl = slice(y0, y0 + n)
+        return n, y0, data["left"][:, sl, :].astype(np.float64), data["right"][:, sl, :].astype(np.float64)
+
+    rows, y0, left, right = band(rows)
+    feats, cost, wx, wy, r = oracle_frame(O, left, right, spec, params, pad)
+    trows, ty0, tl, tr = band(time_rows if time_rows > 0 else rows)
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        oracle_frame(O, tl, tr, spec, params, pad)
+        times.append(time.perf_counter() - t0)
     Wd = left.shape[2]
     energy = O.energy(cost, wx, wy, r["labels"], 0.5, 2.0, 2)
-    return dict(seconds=dt, rows=rows, y0=y0, pd=rows * Wd * D, labels=r["labels"], lb=r["lb"], energy=energy,
-                feat_left=feats[0], cost=cost, left=left.astype(np.float32), right=right.astype(np.float32))
+    return dict(seconds=float(np.median(times)), times=times, time_rows=trows, time_y0=ty0,
+                pd_timed=trows * Wd * spec["D"], rows=rows, y0=y0, pd=rows * Wd * spec["D"], labels=r["labels"],
+                lb=r["lb"], energy=energy, feat_left=feats[0], cost=cost, left=left.astype(np.float32),
+                right=right.astype(np.float32))
+
+
+def time_config(pkg, h, workload, precision, steps, cpu_repeats=0, arch="all3x3", layers_n=7):
+    """One workload's frame (FP32 or BF16) on the GPU: median ms over `steps`
+    frames (HIP events, after a warm-up frame), and optionally the FP64 oracle
+    of the same frame (median of cpu_repeats runs after the GPU one) with the
+    agreement of the labels."""
+    import numpy as np
+    import torch
+    from cnn_crf_stereo_amd import pipeline, synth
+
+    spec = dict(synth.WORKLOADS[workload])
+    data = synth.stereo_pair(workload)
+    layers, pad = synth.architecture(layers_n, spec["C"], arch)
+    params = synth.glorot_weights(layers)
+    fr = pipeline.Frame(h, params, precision, spec["D"], iters=spec["iters"], color=spec["C"] == 3)
+    left = torch.as_tensor(data["left"], device="cuda")
+    right = torch.as_tensor(data["right"], device="cuda")
+    out = fr(left, right)
+    ms = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = fr(left, right)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    H, W, D = spec["H"], spec["W"], spec["D"]
+    res = {"precision": precision, "ms_per_frame": round(float(np.median(ms)), 4),
+           "pixel_disp_per_s": H * W * D / (float(np.median(ms)) / 1e3),
+           "bad_px_gt_gt1": bad_pixel_rate(out["labels"].cpu().numpy(), data["disp"], D)}
+    if cpu_repeats:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        l64, r64 = data["left"].astype(np.float64), data["right"].astype(np.float64)
+        times, r = [], None
+        for _ in range(cpu_repeats + 1):  # the first run is the warm-up
+            t0 = time.perf_counter()
+            _, _, _, _, r = oracle_frame(O, l64, r64, spec, params, pad)
+            times.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(times[1:]))
+        lab = out["labels"].cpu().numpy()
+        res["cpu_oracle"] = {"ms_per_frame": round(cpu_s * 1e3, 1), "runs": cpu_repeats, "cores": cpu_cores(),
+                             "pixel_disp_per_s": H * W * D / cpu_s,
+                             "bad_px_gt_gt1": bad_pixel_rate(r["labels"], data["disp"], D),
+                             "labels_equal_gpu": float((lab == r["labels"]).mean())}
+    return res
+
+
+def cpu_cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
 def stage_errors(ops, h, fr, out, cb):
@@ -420,6 +493,20 @@ def train_bench(pkg, h, workload, layers_n, arch, batch, steps, warmup, iters, p
     return out
 
 
+def train_steps(pkg, h, steps):
+    """The training measurements: 4 x 256x512 (FP32 checkpointed, FP32
+    store-everything, BF16 checkpointed) and 1488x992 D = 256 (BF16 checkpointed)."""
+    return [
+        train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=steps, warmup=1, iters=8),
+        train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=steps, warmup=1, iters=8,
+                    checkpoint=False, grad_check=False),
+        train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=steps, warmup=1, iters=8,
+                    precision="bf16"),
+        train_bench(pkg, h, "rds-1488x992-d256", 7, "all3x3", batch=1, steps=max(1, steps - 1), warmup=1, iters=10,
+                    precision="bf16"),
+    ]
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -432,7 +519,13 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="rows of the CPU-baseline band (0 = the whole frame)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-time-rows", type=int, default=94,
+                    help="rows of the band the CPU baseline is timed on (median of 3 runs)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 320x240 and 744x496 RGB configurations")
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurements")
+    ap.add_argument("--train-only", action="store_true",
+                    help="run only the training-step measurements and print them as one JSON line")
     ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
                     help="2: the right tower runs on a second stream beside the left one")
     ap.add_argument("--inflight", type=int, default=2,
@@ -454,6 +547,10 @@ def main():
     from cnn_crf_stereo_amd import capi, synth
 
     torch.cuda.set_device(0)
+    if args.train_only:
+        h = capi.Handle()
+        print(json.dumps({"train_step": train_steps(pkg, h, args.steps if args.steps < 10 else 3)}))
+        return
     spec = dict(synth.WORKLOADS[args.workload])
     data = synth.stereo_pair(args.workload)
     layers, pad = synth.architecture(args.layers, spec["C"], args.arch)
@@ -564,12 +661,13 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(data, spec, params, pad, args.cpu_rows)
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = dict(value=cb["pd"] / cb["seconds"], unit="pixel*disp/s", cores=cores, kind="port",
-                   sample="FP64 oracle of every stage on a %dx%d band (rows %d-%d) of the same pair, D=%d, "
-                          "%d CRF iterations: %.2f s" % (W, cb["rows"], cb["y0"], cb["y0"] + cb["rows"] - 1, D,
-                                                          spec["iters"], cb["seconds"]))
+        cb = cpu_baseline(data, spec, params, pad, args.cpu_rows, repeats=3, time_rows=args.cpu_time_rows)
+        cpu = dict(value=cb["pd_timed"] / cb["seconds"], unit="pixel*disp/s", cores=cpu_cores(), kind="port",
+                   sample="FP64 oracle of every stage on a %dx%d band (rows %d-%d) of the same pair, D=%d, %d CRF "
+                          "iterations: median of 3 runs after a warm-up run (the whole-frame oracle run), "
+                          "%.2f s (runs %s s)" % (W, cb["time_rows"], cb["time_y0"],
+                                                  cb["time_y0"] + cb["time_rows"] - 1, D, spec["iters"],
+                                                  cb["seconds"], ", ".join("%.2f" % t for t in cb["times"])))
         # The same band through both GPU forms, compared with the oracle's labels.
         from cnn_crf_stereo_amd import ops, pipeline
 
@@ -587,15 +685,42 @@ def main():
         result_extra["bad_px_band_vs_oracle"] = band
 
     if not args.no_train:
-        result_extra["train_step"] = [
-            train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=3, warmup=1, iters=8),
-            train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=3, warmup=1, iters=8,
-                        checkpoint=False, grad_check=False),
-            train_bench(pkg, h, "rds-512x256-d128", 7, "all3x3", batch=4, steps=3, warmup=1, iters=8,
-                        precision="bf16"),
-            train_bench(pkg, h, "rds-1488x992-d256", 7, "all3x3", batch=1, steps=2, warmup=1, iters=10,
-                        precision="bf16"),
-        ]
+        result_extra["train_step"] = train_steps(pkg, h, 3)
+
+    if not args.no_configs:
+        result_extra["configs"] = {
+            "rds-320x240-d32": [time_config(pkg, h, "rds-320x240-d32", "fp32", 10, cpu_repeats=3),
+                                time_config(pkg, h, "rds-320x240-d32", "bf16", 10)],
+            "rds-744x496-rgb-d64": [time_config(pkg, h, "rds-744x496-rgb-d64", "fp32", 10),
+                                    time_config(pkg, h, "rds-744x496-rgb-d64", "bf16", 10)],
+        }
+
+    # Each stage's floor at the chip's peak (dense BF16 MFMA 2.5 PFLOP/s, HBM
+    # 8 TB/s) with its algorithmic work: the conv FLOPs, the correlation's
+    # feature reads and volume write, 3 volume passes per CRF half-step (read C,
+    # read the other family's dual, write the dual), one volume read for the
+    # decode; the frame time as a fraction of their sum.
+    tower_flops = 0.0
+    hh, ww = H + 2 * pad, W + 2 * pad
+    for (K, C, KH, KW) in layers:
+        hh, ww = hh - KH + 1, ww - KW + 1
+        tower_flops += 2.0 * K * C * KH * KW * hh * ww
+    V = 4.0 * H * W * D
+    cs = (layers[-1][0] + 7) // 8 * 8
+    floors = {
+        "conv": 2 * tower_flops / (MFMA_PEAK_TFLOPS["bf16"] * 1e12) * 1e3,
+        "correlation": (2.0 * H * W * cs * 2 + V) / (HBM_PEAK_GBS * 1e9) * 1e3,
+        "crf_sweeps": spec["iters"] * 2 * 3 * V / (HBM_PEAK_GBS * 1e9) * 1e3,
+        "crf_decode": V / (HBM_PEAK_GBS * 1e9) * 1e3,
+        "normalize_edges_remap": (2 * 3 * 4.0 * (H + 2 * pad) * (W + 2 * pad) + 12.0 * H * W + 8.0 * H * W)
+                                 / (HBM_PEAK_GBS * 1e9) * 1e3,
+    }
+    result_extra["frame_roofline"] = {
+        "floor_ms": {k: round(v, 4) for k, v in floors.items()},
+        "sum_floor_ms": round(sum(floors.values()), 4),
+        "frame_ms": round(ms_bf16, 4),
+        "frac": round(sum(floors.values()) / ms_bf16, 4),
+    }
 
     line = {
         "metric": METRIC,
