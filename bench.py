@@ -444,7 +444,10 @@ def train_bench(pkg, h, workload, layers_n, arch, batch, steps, warmup, iters, p
     weights, the towers and the correlation in `precision`, checkpointed towers
     (only the padded images kept; layer outputs recomputed in the backward) or
     store-everything layers.  Reports ms/step, the peak of torch's device
-    allocations over the timed steps, and the relative error (L2 over all
+    allocations over the timed steps (all live tensors of the process, and the
+    part above what was allocated when the timed steps began: the step's
+    activations and gradients; the handle's own scratch is not a torch
+    allocation), and the relative error (L2 over all
     tower and CRF parameters) of the first step's gradient against the FP32
     store-everything backward at the same weights."""
     import numpy as np
@@ -484,12 +487,15 @@ def train_bench(pkg, h, workload, layers_n, arch, batch, steps, warmup, iters, p
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
     t0 = time.perf_counter()
     losses = [AG.training_step(model, h, frames, D, lr=1e-4, iters=iters) for _ in range(steps)]
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    peak = torch.cuda.max_memory_allocated()
     out.update({"ms_per_step": round(ms, 2), "loss": [round(v, 5) for v in losses],
-                "peak_torch_allocated_gb": round(torch.cuda.max_memory_allocated() / 1e9, 3)})
+                "peak_torch_allocated_gb": round(peak / 1e9, 3),
+                "step_peak_above_start_gb": round((peak - base) / 1e9, 3)})
     return out
 
 
