@@ -82,6 +82,50 @@ struct Vec {
   float v[LPL];
 };
 
+// Row sweeps with a message wave (SEG rows): the backward pass keeps every
+// seg_len()-th message in the scratch volume, and a second wave of the
+// workgroup recomputes each segment's messages from those checkpoints into an
+// LDS ring of kRowRing segments, two segments ahead of the forward pass.  The
+// ring holds a node's messages at a stride of 64 * LPL floats.
+constexpr int kRowRing = 3;
+
+// Workgroup barrier that orders LDS accesses only: the waves wait for their
+// own LDS operations (lgkmcnt) but keep their global loads in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// A node's messages in the LDS ring (lane-major, LPL floats per lane; stores
+// of at most 64 bits, as for global memory).
+template <int LPL>
+__device__ __forceinline__ void st_ring(float* p, int lane, const Vec<LPL>& x) {
+  if constexpr (LPL == 4) {
+    *reinterpret_cast<float2*>(p + lane * 4) = make_float2(x.v[0], x.v[1]);
+    *reinterpret_cast<float2*>(p + lane * 4 + 2) = make_float2(x.v[2], x.v[3]);
+  } else if constexpr (LPL == 2) {
+    *reinterpret_cast<float2*>(p + lane * 2) = make_float2(x.v[0], x.v[1]);
+  } else {
+    p[lane] = x.v[0];
+  }
+}
+
+template <int LPL>
+__device__ __forceinline__ Vec<LPL> ld_ring(const float* p, int lane) {
+  Vec<LPL> o;
+  if constexpr (LPL == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p + lane * 4);
+    o.v[0] = t.x; o.v[1] = t.y; o.v[2] = t.z; o.v[3] = t.w;
+  } else if constexpr (LPL == 2) {
+    const float2 t = *reinterpret_cast<const float2*>(p + lane * 2);
+    o.v[0] = t.x; o.v[1] = t.y;
+  } else {
+    o.v[0] = p[lane];
+  }
+  return o;
+}
+
 // Vector shifted by one label: out(s) = x(s - 1) (dir < 0) or x(s + 1) (dir > 0).
 template <int LPL>
 __device__ __forceinline__ Vec<LPL> shift_label(const Vec<LPL>& x, int dir, float fill) {
@@ -424,12 +468,18 @@ __device__ double chain_kept(const float* v, const double* b, int n, int lane, d
 // (clamped to the last node, so it never leaves the chain).  The last partial
 // block is an unrolled tail that issues no loads.
 template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK, bool PE>
-__global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
+__global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 128 : 64) void k_chain(ChainArgs a) {
   extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
   constexpr bool kCols = MODE == kColsMinorant;
+  // Rows with a message wave: wave 0 runs both passes, wave 1 recomputes the
+  // forward pass's messages into the LDS ring (see kRowRing).
+  constexpr bool kRowSeg = !kCols && SEG;
+  constexpr int KR = kRowSeg ? seg_len<LPL>() : 1;
+  static_assert(!kRowSeg || P == kRowRing * KR, "the forward ring spans the message ring");
   static_assert(P <= 64, "one weight per lane and step of a block");
   const int c = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int n = a.n, D = a.D;
   ShiftCarry cc;
   const float L1v = in_vgpr(a.L1), L2v = in_vgpr(a.L2);
@@ -460,8 +510,15 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   float* w_lds = reinterpret_cast<float*>(b_lds + n);
   float* u_lds = w_lds + n;
   float* w2_lds = u_lds + n;  // PE only
+  // kRowSeg: the forward minima go to v_lds (u_lds keeps the backward minima
+  // that the message wave reads), then the message ring at a 16-byte boundary.
+  float* v_lds = w2_lds + (PE ? n : 0);
+  // (offset arithmetic on the LDS pointer itself keeps its address space)
+  float* ring = reinterpret_cast<float*>(b_lds) + (((PE ? 6 : 5) * n + 3) & ~3);
+  constexpr int kNS = 64 * LPL;  // ring stride per node, floats
+  float* fw_lds = kRowSeg ? v_lds : u_lds;
   const long wstep = a.pns * a.wmul;
-  for (int i = lane; i < n - 1; i += 64) {
+  for (int i = threadIdx.x; i < n - 1; i += blockDim.x) {
     w_lds[i] = wb[(long)i * wstep];
     if constexpr (PE) w2_lds[i] = wb2[(long)i * wstep];
   }
@@ -477,7 +534,83 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   const Rsrc rM = chain_rsrc(msg, (long)n * D);
 
   // ---- backward pass: step s handles i = n-2-s: Bh[i] = minconv(theta[i+1] + Bh[i+1]) - min(...)
+  // kRowSeg: wave 0 only; it stores the messages at i = nb - m*KR (m >= 1), so
+  // a step s0 + j of a block (s0 a multiple of P, itself a multiple of KR) is a
+  // checkpoint exactly when (j + 1) % KR == 0.
   const int nb = n - 1;
+  Vec<LPL> bh;
+  bool nan_seen = false;  // NANCHK: a unary read by this lane was NaN
+  double bsum = 0.0;
+  Vec<LPL> a0, c0, th0;
+  // kRowSeg, wave 1: the forward pass's messages.  Segment s holds the forward
+  // steps i' = s*KR + t (t < KR) of node i = i' - g, where the g = M*KR - nb
+  // virtual steps before node 0 end-align the segments with the checkpoints
+  // (Bh at (s+1)*KR - g is one; Bh_{n-1} = 0).  Each message is recomputed
+  // with the backward step's own operations and operands (theta_{i+1} + Bh_{i+1},
+  // the stored minimum u_i, the edge's penalties), so it equals the message
+  // the backward pass formed, and goes to ring slot i' % (kRowRing*KR).
+  // Segments 0 and 1 are done before the first barrier, then segment s before
+  // the barrier that wave 0 passes at the start of segment s - 1.
+  auto row_messages = [&]() {
+    if constexpr (kRowSeg) {
+      constexpr int K = KR;
+      const int M = (nb + K - 1) / K;
+      if (M == 0) return;
+      const int g = M * K - nb;
+      struct HSeg {
+        Vec<LPL> a[K], m;
+        float u, w, w2;
+      };
+      HSeg X, Y;
+      auto load_h = [&](int s, HSeg& B) {  // s is clamped by the caller to a segment
+        const int ib = s * K - g;            // node of t = 0 (negative only in segment 0)
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+          B.a[t] = ld_buf<LPL, FULL>(rA, min(max(ib + t + 1, 1), n - 1) * strideB, lane, D, kInfF);
+        B.m = ld_buf<LPL, FULL>(rM, (ib + K) * D * 4, lane, D, 0.f);
+        const int e = max(min(ib + lane, n - 2), 0);
+        B.u = u_lds[e];
+        B.w = w_lds[e];
+        B.w2 = PE ? w2_lds[e] : 0.f;
+      };
+      auto run_h = [&](int s, const HSeg& B) {
+        const int ib = s * K - g;
+        float* slot = ring + (s % kRowRing) * K * kNS;
+        Vec<LPL> Xb = B.m;
+#pragma unroll
+        for (int t = K - 1; t >= 0; --t) {
+          if (ib + t >= 0) {
+            const Vec<LPL> U = plus(theta_of(B.a[t], B.a[t]), Xb);
+            const float u = lane_weight(B.u, t);
+            const Vec<LPL> m = minconv<LPL, DC, PE>(U, u, lane_weight(B.w, t), PE ? lane_weight(B.w2, t) : 0.f,
+                                                    L1v, L2v, a.delta, cc);
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) Xb.v[k] = valid[k] ? m.v[k] - u : 0.f;
+            st_ring<LPL>(slot + t * kNS, lane, Xb);
+          }
+        }
+      };
+      load_h(0, X);
+      load_h(min(1, M - 1), Y);
+      run_h(0, X);
+      load_h(min(2, M - 1), X);
+      if (M > 1) run_h(1, Y);
+      lds_barrier();  // segments 0 and 1 are in the ring
+      for (int s = 2; s < M;) {
+        load_h(min(s + 1, M - 1), Y);
+        __builtin_amdgcn_sched_barrier(0);
+        run_h(s, X);
+        lds_barrier();
+        if (++s >= M) break;
+        load_h(min(s + 1, M - 1), X);
+        __builtin_amdgcn_sched_barrier(0);
+        run_h(s, Y);
+        lds_barrier();
+        ++s;
+      }
+    }
+  };
+  if (!kRowSeg || wid == 0) {
   auto load_b = [&](int s, Vec<LPL>& av, Vec<LPL>& cv) {
     const int node = s < nb ? n - 1 - s : 1;  // clamped: beyond the chain re-read node 1
     if constexpr (kCols) {
@@ -490,7 +623,6 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   // weights of steps s0 .. s0+63 of the backward pass (edge n-2-s)
   auto wlane_b = [&](int s0) { return w_lds[max(n - 2 - (s0 + lane), 0)]; };
   auto wlane_b2 = [&](int s0) { return PE ? w2_lds[max(n - 2 - (s0 + lane), 0)] : 0.f; };
-  Vec<LPL> bh;
 #pragma unroll
   for (int j = 0; j < LPL; ++j) bh.v[j] = 0.f;
   st_node<LPL, FULL>(msg + (long)(n - 1) * D, lane, D, bh);
@@ -500,7 +632,8 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     for (int k = 0; k < LPL; ++k) U.v[k] = th.v[k] + bh.v[k];
     return U;
   };
-  auto step_b = [&](int s, const Vec<LPL>& U, float wi, float wi2) {
+  // ck: (kRowSeg) this step's message is a checkpoint
+  auto step_b = [&](int s, const Vec<LPL>& U, float wi, float wi2, bool ck) {
     const int i = n - 2 - s;
     const float u = wave_min(local_min(U));
     const Vec<LPL> m = minconv<LPL, DC, PE>(U, u, wi, wi2, L1v, L2v, a.delta, cc);
@@ -509,12 +642,13 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
     if constexpr (kCols) {
       constexpr int K = SEG ? seg_len<LPL>() : 0;
       if (K == 0 || i % K == 0) st_node<LPL, FULL>(msg + (long)i * D, lane, D, bh);  // i is wave-uniform
+    } else if constexpr (kRowSeg) {
+      if (ck) st_buf<LPL, FULL>(rM, i * D * 4, lane, D, bh);
     } else {
       st_buf<LPL, FULL>(rM, i * D * 4, lane, D, bh);
     }
     u_lds[i] = u;  // every lane writes the same value
   };
-  bool nan_seen = false;  // NANCHK: a unary read by this lane was NaN
   if (nb > 0) {
 #pragma unroll
     for (int j = 0; j < P; ++j) load_b(j, ra[j], rc[j]);
@@ -523,6 +657,9 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   float wv = wlane_b(0), wv2 = wlane_b2(0);
   for (; s0 + P <= nb; s0 += P) {
     const float wnext = wlane_b(s0 + P), wnext2 = wlane_b2(s0 + P);
+    // kRowSeg: byte offsets of the block's refills, one per lane (v_readlane
+    // per step instead of clamping each offset with scalar instructions)
+    const int bo = kRowSeg ? (s0 + P + lane < nb ? n - 1 - (s0 + P + lane) : 1) * strideB : 0;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
@@ -530,8 +667,11 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       // refill reuses the slot's registers.
       const Vec<LPL> U = plus_bh(theta_of(ra[j], rc[j]));
       __builtin_amdgcn_sched_barrier(kSchedValu);
-      load_b(s0 + j + P, ra[j], rc[j]);
-      step_b(s0 + j, U, lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f);
+      if constexpr (kRowSeg)
+        ra[j] = ld_buf<LPL, FULL>(rA, __builtin_amdgcn_readlane(bo, j), lane, D, kInfF);
+      else
+        load_b(s0 + j + P, ra[j], rc[j]);
+      step_b(s0 + j, U, lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f, (j + 1) % KR == 0);
     }
     wv = wnext;
     wv2 = wnext2;
@@ -540,21 +680,28 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
   for (int j = 0; j < P; ++j)
     if (s0 + j < nb) {
       if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
-      step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])), lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f);
+      step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])), lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f,
+             (j + 1) % KR == 0);
     }
-  const double bsum = chain_offsets(u_lds, b_lds, n, lane);  // b_0 = sum of all minima
+  bsum = chain_offsets(u_lds, b_lds, n, lane);  // b_0 = sum of all minima
 
   // bh holds Bh[0]; theta[0] is read directly.
-  const Vec<LPL> a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
+  a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
   if constexpr (NANCHK) {
     nan_seen |= any_nan(a0);
     const bool any = __ballot(nan_seen) != 0;  // all lanes take part
     if (lane == 0) a.nan_flags[c] = any ? 1 : 0;
   }
-  Vec<LPL> c0;
   if constexpr (kCols) c0 = ld_node<LPL, FULL>(Cb, lane, D, kInfF);
-  const Vec<LPL> th0 = theta_of(a0, c0);
-  __syncthreads();
+  th0 = theta_of(a0, c0);
+  }
+  __syncthreads();  // kRowSeg: the checkpoints and u_lds are now visible to wave 1
+  if constexpr (kRowSeg) {
+    if (wid == 1) {
+      row_messages();
+      return;
+    }
+  }
   Vec<LPL> M0;
 #pragma unroll
   for (int k = 0; k < LPL; ++k) M0.v[k] = th0.v[k] + bh.v[k];
@@ -590,7 +737,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         V.v[k] = valid[k] ? R.v[k] - gt.v[k] : kInfF;
       }
       const float v = wave_min(local_min(V));
-      u_lds[i] = v;  // every lane writes the same value
+      fw_lds[i] = v;  // every lane writes the same value
       const Vec<LPL> m = minconv<LPL, DC, PE>(V, v, wi, wi2, L1v, L2v, a.delta, cc);
       Vec<LPL> mm;
 #pragma unroll
@@ -613,6 +760,76 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
       if constexpr (kCols) st_node<LPL, FULL>(Aw + i * nstride, lane, D, out);
       else st_buf<LPL, FULL>(rW, i * strideB, lane, D, out);
     };
+    if constexpr (kRowSeg) {
+      // Forward step i' = i + g (see row_messages): A_{i+1} in register ring
+      // slot i' % P (P = kRowRing*KR, so the segments i' = q*KR .. q*KR+KR-1
+      // line up with the blocks), Bh_i from LDS ring slot i' % P, read two
+      // steps ahead.  Segments 0 and 1 are complete at the first barrier;
+      // before the first step of segment q (1 <= q <= M-2) a barrier pairs
+      // with wave 1's barrier after segment q+1, which also frees the slots of
+      // segment q-1 for segment q+2.
+      constexpr int K = KR;
+      const int M = (nb + K - 1) / K;
+      const int g = M * K - nb;
+      const int nbp = M * K;
+      auto load_a = [&](int ip, Vec<LPL>& av) {
+        const int node = max(min(ip - g, nb - 1), 0) + 1;
+        av = ld_buf<LPL, FULL>(rA, node * strideB, lane, D, kInfF);
+      };
+      // Byte offsets of a block's refills (node of step ip0 + P + lane), one per
+      // lane: a step takes its offset with v_readlane instead of clamping it.
+      auto refill_offsets = [&](int ip0) { return (max(min(ip0 + P - g + lane, nb - 1), 0) + 1) * strideB; };
+      auto wl = [&](int ip0) { return w_lds[max(min(ip0 - g + lane, n - 2), 0)]; };
+      auto wl2 = [&](int ip0) { return PE ? w2_lds[max(min(ip0 - g + lane, n - 2), 0)] : 0.f; };
+      Vec<LPL> bm[3];
+      if (nb > 0) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) load_a(j, ra[j]);
+        lds_barrier();  // segments 0 and 1 are in the ring
+        bm[0] = ld_ring<LPL>(ring, lane);
+        bm[1] = ld_ring<LPL>(ring + kNS, lane);
+      }
+      float wf = wl(0), wf2 = wl2(0);
+      // Step j of the block at ip0.  LOW: the block may start with virtual
+      // steps (i < 0, skipped); TAIL: no refills, steps past the chain skipped.
+      auto fstep = [&](int ip0, int ao, int j, bool LOW, bool TAIL) {
+        const int ip = ip0 + j;
+        if (TAIL && ip >= nbp) return;
+        if (j % K == 0) {
+          const int q = ip / K;
+          if (q >= 1 && q <= M - 2) lds_barrier();
+        }
+        const int i = ip - g;
+        const Vec<LPL> an = fresh(ra[j]);
+        const Vec<LPL> RB = plus(R, bm[j % 3]);
+        __builtin_amdgcn_sched_barrier(kSchedValu);
+        if (!TAIL) ra[j] = ld_buf<LPL, FULL>(rA, __builtin_amdgcn_readlane(ao, j), lane, D, kInfF);
+        bm[(j + 2) % 3] = ld_ring<LPL>(ring + ((j + 2) % P) * kNS, lane);
+        if (!LOW || i >= 0)
+          store_f(i, step_f(i, RB, an, theta_of(an, an), lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f));
+      };
+      int i0p = 0;
+      if (nbp >= P) {
+        {
+          const float wn = wl(P), wn2 = wl2(P);
+          const int ao = refill_offsets(0);
+#pragma unroll
+          for (int j = 0; j < P; ++j) fstep(0, ao, j, true, false);
+          wf = wn;
+          wf2 = wn2;
+        }
+        for (i0p = P; i0p + P <= nbp; i0p += P) {
+          const float wn = wl(i0p + P), wn2 = wl2(i0p + P);
+          const int ao = refill_offsets(i0p);
+#pragma unroll
+          for (int j = 0; j < P; ++j) fstep(i0p, ao, j, false, false);
+          wf = wn;
+          wf2 = wn2;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < P; ++j) fstep(i0p, 0, j, true, true);
+    } else
     if constexpr (kCols && SEG && seg_len<LPL>() > 0) {
       // Segments of K forward steps i0 .. i0+K-1.  Their nodes i0+1 .. i0+K
       // (unaries and costs), the checkpoint Bh_{min(i0+K, n-1)} (Bh_{n-1} = 0
@@ -731,7 +948,7 @@ __global__ __launch_bounds__(64) void k_chain(ChainArgs a) {
         out.v[k] = MODE == kRowsMinorant ? a_cur.v[k] - R.v[k] : a_cur.v[k] + R.v[k];
       st_node<LPL, FULL>(Aw + (n - 1) * nstride, lane, D, out);
     }
-    const double kept = chain_kept(u_lds, b_lds, n, lane, kGammaD);
+    const double kept = chain_kept(fw_lds, b_lds, n, lane, kGammaD);
     if (lane == 0) {
       a.lam[c] = Lambda;
       a.kept[c] = kept;
@@ -940,10 +1157,19 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
 
 template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK, bool PE>
 static void launch_sweep(ChainArgs& a, size_t lds, hipStream_t st) {
+  constexpr bool kRowSeg = MODE != kColsMinorant && SEG;
+  if constexpr (kRowSeg)  // v_lds, the alignment of the ring, and the ring
+    lds += sizeof(float) * (size_t)a.n + 16 + sizeof(float) * kRowRing * seg_len<LPL>() * 64 * LPL;
   if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
     (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK, PE>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK, PE><<<a.n_chains, 64, lds, st>>>(a);
+  k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK, PE><<<a.n_chains, kRowSeg ? 128 : 64, lds, st>>>(a);
+}
+
+// Register-ring depth of the row sweeps with a message wave: three segments.
+template <int LPL>
+constexpr int row_seg_p() {
+  return kRowRing * seg_len<LPL>();
 }
 
 template <int LPL, int MODE, int P, bool FULL, int DC, bool PE>
@@ -959,6 +1185,11 @@ static void launch_one(ChainArgs& a, size_t lds, bool seg, hipStream_t st) {
       launch_sweep<LPL, MODE, P, FULL, DC, true, false, PE>(a, lds, st);
     else
       launch_sweep<LPL, MODE, P, FULL, DC, false, false, PE>(a, lds, st);
+  } else if (seg) {
+    if (a.nan_flags)
+      launch_sweep<LPL, MODE, row_seg_p<LPL>(), FULL, DC, true, true, PE>(a, lds, st);
+    else
+      launch_sweep<LPL, MODE, row_seg_p<LPL>(), FULL, DC, true, false, PE>(a, lds, st);
   } else if (a.nan_flags) {
     launch_sweep<LPL, MODE, P, FULL, DC, false, true, PE>(a, lds, st);
   } else {
@@ -1027,14 +1258,17 @@ int launch_chain(ChainMode mode, ChainArgs& a, hipStream_t st) {
       return fail(CCS_ERR_ARG, "row chain of %d nodes x %d labels exceeds 2 GiB", a.n, a.D);
   }
   bool seg = false;
+  const long vol = (long)a.n_chains * a.n * a.D * (long)sizeof(float);
   if (mode == kColsMinorant && kSeg > 0) {
     // A segment's nodes are addressed with 32-bit byte offsets from its first node.
     const long span = ((long)(kSeg - 1) * a.pns * a.D + a.D) * (long)sizeof(float);
-    const long vol = (long)a.n_chains * a.n * a.D * (long)sizeof(float);
     seg = span <= 0x7fffffffL && vol >= seg_min_bytes();
   }
+  // Row sweeps take the message wave at the same volumes (their chains are
+  // addressed within the 2 GiB checked above).
+  const bool row_seg = kSeg > 0 && vol >= seg_min_bytes();
   switch (mode) {
-    case kRowsMinorant: return launch_mode<kRowsMinorant>(a, false, st);
+    case kRowsMinorant: return launch_mode<kRowsMinorant>(a, row_seg, st);
     case kColsMinorant: return launch_mode<kColsMinorant>(a, seg, st);
     case kRowsDecode: return launch_mode<kRowsDecode>(a, false, st);
   }

@@ -80,8 +80,17 @@ def test_first_row_sweep_reads_cost(handle, H, W, D):
 
 # Shapes below the checkpointing threshold, covering the kernel variants that the
 # large benchmark volumes do not: labels that do not fill a lane's slots
-# (D = 37, 100, 200), delta != 2, and 1, 2 and 4 labels per lane.
-SMALL = [(97, 131, 100, 2), (120, 90, 37, 3), (64, 70, 256, 4), (150, 257, 200, 2), (61, 97, 64, 2)]
+# (D = 37, 100, 200), delta != 2, and 1, 2 and 4 labels per lane.  The row
+# sweeps' message wave splits a chain of W - 1 steps into segments of 12 steps
+# (6 at 4 labels per lane), end-aligned: the widths give every remainder class
+# used below, chains of one node, of less than one segment and of one to two
+# segments.
+SMALL = [(97, 131, 100, 2), (120, 90, 37, 3), (64, 70, 256, 4), (150, 257, 200, 2), (61, 97, 64, 2),
+         (30, 1, 32, 2), (40, 9, 64, 2), (33, 20, 128, 2), (17, 25, 128, 2), (9, 38, 256, 2), (12, 14, 64, 3)]
+# Per-edge penalties (two levels) and a whole ccs_slackprop run, whose first row
+# sweep also checks the costs for NaN.
+SMALL_PE = [(23, 131, 128), (11, 40, 256), (7, 18, 64)]
+SMALL_RUN = [(41, 157, 128), (9, 26, 64)]
 
 _CHILD = r"""
 import sys, numpy as np, torch
@@ -91,11 +100,18 @@ from cnn_crf_stereo_amd import capi
 from test_gpu_crf_determinism import SMALL, small_case, run_small
 h = capi.Handle()
 out = {}
+from test_gpu_crf_determinism import SMALL_PE, SMALL_RUN, pe_case, run_pe, run_full
 for (H, W, D, delta) in SMALL:
     A, sums, labels = run_small(h, *small_case(H, W, D), delta)
     out["%d_%d_%d_%d_A" % (H, W, D, delta)] = A.cpu().numpy()
     out["%d_%d_%d_%d_s" % (H, W, D, delta)] = sums.cpu().numpy()
     out["%d_%d_%d_%d_l" % (H, W, D, delta)] = labels.cpu().numpy()
+for (H, W, D) in SMALL_PE:
+    for k, v in run_pe(h, *pe_case(H, W, D)).items():
+        out["pe_%d_%d_%d_%s" % (H, W, D, k)] = v
+for (H, W, D) in SMALL_RUN:
+    for k, v in run_full(h, *small_case(H, W, D)).items():
+        out["run_%d_%d_%d_%s" % (H, W, D, k)] = v
 np.savez(sys.argv[2], **out)
 print("child ok")
 """
@@ -125,10 +141,36 @@ def run_small(h, cost, wx, wy, delta):
     return A, sums, labels
 
 
-def test_checkpointed_columns_equal_stored_messages(handle, tmp_path):
-    """The column sweeps' checkpointed scheme (forced on by the threshold
-    override in a child process) gives the store-everything scheme's unaries,
-    sums and labels bit for bit, on shapes only the latter runs by default."""
+def pe_case(H, W, D):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(H * 5 + W + D)
+    cost = -torch.softmax(torch.randn(H, W, D, device="cuda", generator=g) * 3, dim=-1)
+    p1 = torch.rand(2, H, W, device="cuda", generator=g) * 0.7 + 0.1
+    p2 = p1 + torch.rand(2, H, W, device="cuda", generator=g) * 2.0 + 0.2
+    return cost, torch.cat([p1, p2]).contiguous()
+
+
+def run_pe(h, cost, pw):
+    pen, bad = ops.pairwise_penalties(h, pw)
+    assert not bad.any()
+    g = ops.slackprop_pw(h, cost, pen, 2, 3, want_unaries=True)
+    torch.cuda.synchronize()
+    return {k: g[k].cpu().numpy() for k in ("labels", "sums", "lb", "unaries")}
+
+
+def run_full(h, cost, wx, wy):
+    g = ops.slackprop(h, cost, wx, wy, 0.5, 2.0, 2, 3, want_unaries=True)
+    ops.slackprop_status(h)
+    torch.cuda.synchronize()
+    return {k: g[k].cpu().numpy() for k in ("labels", "sums", "lb", "unaries")}
+
+
+def test_checkpointed_sweeps_equal_stored_messages(handle, tmp_path):
+    """The checkpointed schemes (forced on by the threshold override in a child
+    process: columns recompute their messages in the forward pass, rows in a
+    second wave) give the store-everything schemes' unaries, sums and labels
+    bit for bit, on shapes only the latter run by default -- plain sweeps,
+    per-edge penalties and whole runs."""
     import os
     import subprocess
     import sys
@@ -148,3 +190,59 @@ def test_checkpointed_columns_equal_stored_messages(handle, tmp_path):
         assert np.array_equal(got[key + "A"].view(np.int32), A.cpu().numpy().view(np.int32)), key
         assert np.array_equal(got[key + "s"], sums.cpu().numpy()), key
         assert np.array_equal(got[key + "l"], labels.cpu().numpy()), key
+    for (H, W, D) in SMALL_PE:
+        ref = run_pe(handle, *pe_case(H, W, D))
+        for k, v in ref.items():
+            key = "pe_%d_%d_%d_%s" % (H, W, D, k)
+            assert np.array_equal(got[key].view(np.uint8), v.view(np.uint8)), key
+    for (H, W, D) in SMALL_RUN:
+        ref = run_full(handle, *small_case(H, W, D))
+        for k, v in ref.items():
+            key = "run_%d_%d_%d_%s" % (H, W, D, k)
+            assert np.array_equal(got[key].view(np.uint8), v.view(np.uint8)), key
+
+
+_CHILD_FULL = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, sys.argv[1])
+import conftest  # loads the package
+from cnn_crf_stereo_amd import capi
+from test_gpu_crf_determinism import full_case, run_sequence
+h = capi.Handle()
+A, sums, labels = run_sequence(h, *full_case())
+np.savez(sys.argv[2], A=A.cpu().numpy(), s=sums.cpu().numpy(), l=labels.cpu().numpy())
+print("child ok")
+"""
+
+
+def full_case():
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4242)
+    H, W, D = 375, 1242, 128
+    cost = -torch.softmax(torch.randn(H, W, D, device="cuda", generator=g) * 3, dim=-1)
+    wx = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    wy = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    return cost, wx, wy
+
+
+def test_full_size_checkpointed_equal_stored_messages(handle, tmp_path):
+    """At 1242 x 375 x 128 (checkpointed rows and columns by default) the
+    sweeps equal the store-everything schemes, forced on in a child process by
+    a threshold above the volume, bit for bit."""
+    import os
+    import subprocess
+    import sys
+
+    import numpy as np
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = str(tmp_path / "full.npz")
+    env = dict(os.environ, CCS_CRF_SEG_MIN_BYTES=str(1 << 40))
+    r = subprocess.run([sys.executable, "-c", _CHILD_FULL, here, path], env=env, cwd=here, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    got = np.load(path)
+    A, sums, labels = run_sequence(handle, *full_case())
+    assert np.array_equal(got["A"].view(np.int32), A.cpu().numpy().view(np.int32))
+    assert np.array_equal(got["s"], sums.cpu().numpy())
+    assert np.array_equal(got["l"], labels.cpu().numpy())
