@@ -4,21 +4,27 @@
 // Feature maps are channel-last, H x W x Cs with Cs = C rounded up to 8, so the
 // im2col row of an output pixel is KH runs of KW * Cs contiguous elements and
 // every 8-element group (16 bytes in bf16) stays inside one tap.  The
-// reduction index is k = tap * Cs + c with tap = i * KW + j, and the packed
-// weight holds W[n][c][KH-1-i][KW-1-j] at k (the flip of true convolution).
+// reduction index runs over (tap, c) with tap = i * KW + j, and the packed
+// weight holds W[n][c][KH-1-i][KW-1-j] there (the flip of true convolution);
+// the bf16 packing orders it by channel stage (see stage_count below).
 //
-// k_conv_bf16: implicit GEMM on v_mfma_f32_16x16x32_bf16.  A workgroup of 4
-//   waves computes a 4-row x 64-pixel output tile for all (<= 16*NT) output
-//   channels; the (4+KH-1) x (64+KW-1) input halo tile sits in LDS, packed
-//   weights stream through LDS in chunk groups; each wave owns one output row
-//   (4 M-tiles x NT N-tiles of accumulators).  Epilogue: + bias, tanh, bf16,
-//   staged through LDS for 208-byte-per-pixel coalesced stores.
+// k_conv_bf16_st: the 100-channel layers (3x3 and 2x2) on
+//   v_mfma_f32_16x16x32_bf16: a workgroup of 4 waves computes a 4-row x
+//   64-pixel output tile for all 112 (padded) output channels, the input halo
+//   tile staged in LDS one channel half at a time, the weights streaming
+//   through a ring of chunk slots filled by direct global->LDS loads.
+//   Epilogue: + bias (the accumulators' start value), tanh, bf16, stored from
+//   the accumulators.
+// k_conv_bf16: the same GEMM for the other channel counts (64 and 8): 4-row x
+//   TW-pixel tiles, the whole input tile in LDS, weights in chunk groups
+//   staged through registers, output staged through LDS.
 // k_conv_f32: the same on v_mfma_f32_16x16x4_f32 (2-row tile, float32 maps).
 // k_conv_generic: VALU direct convolution for the 1- and 3-channel input layer
 //   and any shape the MFMA kernels do not cover.
 #include <cmath>
 
 #include <mutex>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -43,6 +49,20 @@ __device__ __forceinline__ float flipped_weight(const float* __restrict__ w, int
   return w[(((long)n * C + c) * KH + (KH - 1 - i)) * KW + (KW - 1 - j)];
 }
 
+// Channel stages of the bf16 layout.  The 13 eight-channel groups of the
+// 100-channel maps are split 7 + 6 (a workgroup stages half of the input
+// channels at a time); other channel counts form one stage.  Chunk q of stage s
+// holds the stage's groups kg = 4 (q - q0(s)) + 0..3 in tap-major order
+// (kg = tap * ng(s) + g, channel group cg0(s) + g); groups past taps * ng(s) are
+// zero padding.  With one stage this is the plain order k = tap * Cs + c.
+__host__ __device__ constexpr int stage_count(int CG) { return CG == 13 ? 2 : 1; }
+__host__ __device__ constexpr int stage_ng(int CG, int s) { return CG == 13 ? (s == 0 ? 7 : 6) : CG; }
+__host__ __device__ constexpr int stage_cg0(int CG, int s) { return CG == 13 && s == 1 ? 7 : 0; }
+__host__ __device__ constexpr int stage_nq(int CG, int taps, int s) { return (taps * stage_ng(CG, s) + 3) / 4; }
+__host__ __device__ constexpr int packed_chunks(int CG, int taps) {
+  return stage_nq(CG, taps, 0) + (stage_count(CG) > 1 ? stage_nq(CG, taps, 1) : 0);
+}
+
 __global__ void k_pack_bf16(const float* __restrict__ w, int K, int C, int Cs, int KH, int KW, int NQ, int NP,
                             bf16* __restrict__ out) {
   const long total = (long)NQ * NP * RPB;
@@ -51,8 +71,12 @@ __global__ void k_pack_bf16(const float* __restrict__ w, int K, int C, int Cs, i
   const int kk = (int)(idx % RPB);
   const int n = (int)((idx / RPB) % NP);
   const int q = (int)(idx / ((long)RPB * NP));
-  const int k = q * 32 + bswz(n, kk / 8) * 8 + kk % 8;  // the swizzle is an involution
-  const int tap = k / Cs, c = k % Cs;
+  const int CG = Cs / 8, taps = KH * KW;
+  const int st = stage_count(CG) > 1 && q >= stage_nq(CG, taps, 0) ? 1 : 0;
+  const int ng = stage_ng(CG, st);
+  const int kg = 4 * (q - (st ? stage_nq(CG, taps, 0) : 0)) + bswz(n, kk / 8);  // the swizzle is an involution
+  const int tap = kg < taps * ng ? kg / ng : taps;  // taps: a padding group (zero)
+  const int c = (stage_cg0(CG, st) + kg % ng) * 8 + kk % 8;
   out[idx] = f32_to_bf16(flipped_weight(w, n, c, tap, K, C, KH, KW));
 }
 
@@ -96,6 +120,7 @@ struct Bf16Cfg {
   static constexpr int B_VPT = (QG * chunk_bytes / 16 + 255) / 256;
   static constexpr int lds_bytes = a_bytes + QG * chunk_bytes;
   static constexpr int out_bytes = kTH * TW * Cs * 2;
+  static_assert(stage_count(CG) == 1, "the channel-staged layout has its own kernel");
   static_assert(QG_max >= 1, "conv tile does not fit in LDS");
   static_assert(out_bytes <= a_bytes, "output staging must fit in the input tile");
 };
@@ -268,6 +293,217 @@ __global__ __launch_bounds__(256, MINW) void k_conv_bf16(const bf16* __restrict_
 #undef CCS_B_NVEC
 #undef CCS_B_LOAD
 #undef CCS_B_STORE
+
+// ---------------------------------------------------------------------------
+// BF16 MFMA implicit GEMM for the 100-channel layers (13 channel groups, two
+// channel stages of the packed layout).
+//
+// A workgroup of 4 waves computes a 4-row x 64-pixel output tile for all 112
+// (padded) output channels; wave w owns output row w: four 16-pixel M-tiles x
+// seven 16-channel N-tiles of accumulators.  Per 32-deep reduction chunk a wave
+// reads 4 input + 7 weight fragments of 1 KB from LDS for 28 MFMAs (the
+// 32-pixel tile of k_conv_bf16 reads 2 + 7 for 14).  The (4+KH-1) x (64+KW-1)
+// input halo tile is staged one channel stage at a time (groups 0-6, then 7-12,
+// at a 112-byte pixel pitch, 44 KB), and the weights stream through a ring of
+// four 8 KB chunk slots filled by direct global->LDS loads, so two 76 KB
+// workgroups share a CU and no staging passes through registers.
+//
+// Ring protocol (chunk c in slot c % 4): the loads of chunk c are issued after
+// the barrier that ends iteration c-4 (all reads of chunk c-4's slot are done by
+// then); every wave waits for its loads of chunk c+2 (vmcnt: only chunk c+3's
+// two loads may be younger) before the barrier that ends iteration c, so chunk
+// c+2 is complete for every wave when iteration c+1 reads its fragments.
+template <int KH, int KW>
+struct StCfg {
+  static constexpr int CG = 13, NT = 7, NP = 16 * NT, TW = 64, MT = TW / 16;
+  static constexpr int taps = KH * KW;
+  static constexpr int PG = stage_ng(CG, 0);  // 16-byte groups per LDS pixel (the larger stage)
+  static constexpr int NQ0 = stage_nq(CG, taps, 0), NQ = packed_chunks(CG, taps);
+  static constexpr int AR = kTH + KH - 1, AC = TW + KW - 1;
+  static constexpr int a_vec = AR * AC * PG;
+  static constexpr int A_RND = (a_vec + 255) / 256;  // direct loads per wave per stage
+  static constexpr int a_bytes = A_RND * 256 * 16;   // the tile, padded to whole rounds
+  static constexpr int chunk_vec = NP * RPB * 2 / 16;  // 448 vectors of 16 bytes
+  static constexpr int slot_bytes = 8192;              // two loads of 64 vectors per wave
+  static constexpr int NSLOT = 4;
+  static constexpr int zero_off = a_bytes + NSLOT * slot_bytes;  // 16 zero bytes (padding groups read them)
+  static constexpr int lds_bytes = zero_off + 16;
+  static constexpr int Cs = CG * 8;
+  static_assert(chunk_vec <= slot_bytes / 16, "a chunk fits its slot");
+};
+
+// Direct global->LDS load of 16 bytes per lane: lane i of the wave writes
+// dst + 16 i (dst is wave-uniform).
+__device__ __forceinline__ void load_lds16(const void* src, void* dst) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+// s_waitcnt with vmcnt = N and the other counters left alone (gfx9 encoding:
+// vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] at [15:14]).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// Barrier after which the LDS written before it (by direct loads whose vmcnt
+// the caller has waited for, or by stores) is visible to every wave.
+__device__ __forceinline__ void tile_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int KH, int KW>
+__global__ __launch_bounds__(256, 2) void k_conv_bf16_st(const bf16* __restrict__ x, int Hin, int Win,
+                                                        const bf16* __restrict__ wpk, const float* __restrict__ bias,
+                                                        int K, int act, bf16* __restrict__ y, int Ho, int Wo) {
+  using Cfg = StCfg<KH, KW>;
+  constexpr int Cs = Cfg::Cs, MT = Cfg::MT, NT = Cfg::NT, PG = Cfg::PG, AC = Cfg::AC;
+  extern __shared__ __align__(16) unsigned char lds[];
+  unsigned char* As = lds;
+  unsigned char* Bs = lds + Cfg::a_bytes;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int lrow = lane & 15, lgrp = lane >> 4;
+  const int w0 = blockIdx.x * Cfg::TW, h0 = blockIdx.y * kTH;
+  const u32x4* wsrc = reinterpret_cast<const u32x4*>(wpk);
+  constexpr int wvec_last = Cfg::NQ * Cfg::chunk_vec - 1;
+
+  // Chunk c's weights into slot c % 4: this wave's two rows of 64 vectors
+  // (vectors past the chunk's 448 read the next chunk, clamped to the buffer,
+  // into the slot's padding).
+  auto load_chunk = [&](int c) {
+    if (c < Cfg::NQ) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int v = (2 * wave + i) * 64 + lane;
+        load_lds16(wsrc + min(c * Cfg::chunk_vec + v, wvec_last),
+                   Bs + (c % Cfg::NSLOT) * Cfg::slot_bytes + (2 * wave + i) * 1024);
+      }
+    }
+  };
+  // Stage st's input tile: vector v = (row, pixel, group) of the stage, pixel
+  // pitch PG groups; addresses clamped to the map (the clamped values feed only
+  // output pixels past Ho x Wo), vectors past the tile land in its padding.
+  auto load_tile = [&](int st) {
+    const int cg0 = stage_cg0(Cfg::CG, st);
+#pragma unroll
+    for (int i = 0; i < Cfg::A_RND; ++i) {
+      const int v = min((i * 4 + wave) * 64 + lane, Cfg::a_vec - 1);
+      const int r = v / (AC * PG), px = (v / PG) % AC, g = v % PG;
+      const int hi = min(h0 + r, Hin - 1), wi = min(w0 + px, Win - 1);
+      const bf16* src = x + (((long)hi * Win + wi) * Cs + min(cg0 + g, Cfg::CG - 1) * 8);
+      load_lds16(src, As + (i * 4 + wave) * 1024);
+    }
+  };
+
+#pragma unroll
+  for (int c = 0; c < Cfg::NSLOT; ++c) load_chunk(c);
+  load_tile(0);
+  if (tid == 0) *reinterpret_cast<u32x4*>(lds + Cfg::zero_off) = u32x4{0u, 0u, 0u, 0u};
+
+  // Accumulators start at the bias (transposed block: lane (lrow, lgrp) holds
+  // pixel 16 m + lrow, channels 16 t + 4 lgrp .. +3).
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    f32x4 bn;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int n = 16 * t + 4 * lgrp + e;
+      bn[e] = (n < K && bias) ? bias[n] : 0.f;
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[m][t] = bn;
+  }
+  wait_vmcnt<0>();
+  tile_barrier();
+
+  // Fragments of chunk q (stage NGS groups, first chunk q0).  A padding group
+  // reads the zero vector: the choice is made on the address, so nothing waits
+  // for the loaded values before they feed the next chunk's MFMAs.
+  auto load_frags = [&](auto ngc, int q, int q0, bf16x8 (&a)[MT], bf16x8 (&b)[NT]) {
+    constexpr int NGS = decltype(ngc)::value;
+    const int kg = 4 * (q - q0) + lgrp;
+    const bool kvalid = kg < Cfg::taps * NGS;
+    const int tap = kvalid ? kg / NGS : 0, g = kvalid ? kg % NGS : 0;
+    const int ti = tap / KW, tj = tap % KW;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int off = (((wave + ti) * AC + 16 * m + lrow + tj) * PG + g) * 16;
+      a[m] = *reinterpret_cast<const bf16x8*>(lds + (kvalid ? off : Cfg::zero_off));
+    }
+    const unsigned char* slot = Bs + (q % Cfg::NSLOT) * Cfg::slot_bytes;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int n = 16 * t + lrow;
+      b[t] = *reinterpret_cast<const bf16x8*>(slot + ((n * RPB + 8 * bswz(n, lgrp)) * 2));
+    }
+  };
+
+  // One chunk: chunk q+1's fragments into (an, bn) (unless q is the stage's
+  // last), chunk q's MFMAs from (a, b), then the ring step.
+  auto chunk_step = [&](auto ngc, int q, int q0, int q1, bf16x8 (&a)[MT], bf16x8 (&b)[NT], bf16x8 (&an)[MT],
+                        bf16x8 (&bn)[NT]) {
+    if (q + 1 < q1) load_frags(ngc, q + 1, q0, an, bn);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[m], acc[m][t], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    wait_vmcnt<2>();  // this wave's loads of chunk q+2 are done
+    tile_barrier();
+    load_chunk(q + Cfg::NSLOT);
+  };
+  // One channel stage: chunks q0 .. q1-1, two per iteration with the fragment
+  // registers swapping roles (no copies between chunks).
+  auto run_stage = [&](auto ngc, int q0, int q1) {
+    bf16x8 a0[MT], b0[NT], a1[MT], b1[NT];
+    load_frags(ngc, q0, q0, a0, b0);
+    int q = q0;
+    for (; q + 1 < q1; q += 2) {
+      chunk_step(ngc, q, q0, q1, a0, b0, a1, b1);
+      chunk_step(ngc, q + 1, q0, q1, a1, b1, a0, b0);
+    }
+    if (q < q1) chunk_step(ngc, q, q0, q1, a0, b0, a1, b1);
+  };
+  run_stage(std::integral_constant<int, stage_ng(13, 0)>{}, 0, Cfg::NQ0);
+  load_tile(1);  // every wave has passed the barrier after its last stage-0 read
+  wait_vmcnt<0>();
+  tile_barrier();
+  run_stage(std::integral_constant<int, stage_ng(13, 1)>{}, Cfg::NQ0, Cfg::NQ);
+  wait_vmcnt<0>();
+
+  // Epilogue: + tanh, bf16, stored straight from the accumulators: a lane
+  // holds four consecutive channels of one pixel (8 bytes); the seven N-tiles
+  // of a pixel fill its 208-byte row, which the L2 merges before writing back.
+  // The padding channels K <= n < Cs come out as exact zeros (zero weights and
+  // bias, tanh(0) = 0).
+  const int ho = h0 + wave;
+  auto epilogue = [&](auto f) {
+    if (ho >= Ho) return;
+    bf16* yrow = y + (long)ho * Wo * Cs;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int wo = w0 + 16 * m + lrow;
+      if (wo >= Wo) continue;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int n0 = 16 * t + 4 * lgrp;
+        typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4;
+        const f32x4 r = {f(acc[m][t][0]), f(acc[m][t][1]), f(acc[m][t][2]), f(acc[m][t][3])};
+        const bf16x4 v = __builtin_convertvector(r, bf16x4);
+        if (n0 < Cs) *reinterpret_cast<bf16x4*>(yrow + (long)wo * Cs + n0) = v;
+      }
+    }
+  };
+  if (act)
+    epilogue([](float v) { return tanh_for_bf16(v); });
+  else
+    epilogue([](float v) { return v; });
+}
 
 // ---------------------------------------------------------------------------
 // FP32 MFMA implicit GEMM (v_mfma_f32_16x16x4_f32)
@@ -521,11 +757,12 @@ struct ConvShape {
   int H, W, C, K, KH, KW;
 };
 
-// Tile width, LDS budget per workgroup and minimum waves per SIMD of the bf16
-// kernels, per filter size, as measured on 1242 x 375 x 100-channel layers:
-// 3x3 -- 32-pixel tiles, two 80 KB workgroups per CU (0.171 ms per layer;
-// 16-pixel tiles at three waves per SIMD: 0.192 ms); 2x2 -- 16-pixel tiles,
-// three 52000-byte workgroups per CU (0.109 ms; 32-pixel tiles: 0.123 ms).
+// Tile width, LDS budget per workgroup and minimum waves per SIMD of
+// k_conv_bf16, per filter size (chosen when it also ran the 100-channel layers,
+// measured on 1242 x 375 x 100-channel layers: 3x3 -- 32-pixel tiles, two 80 KB
+// workgroups per CU, 0.171 ms per layer, 16-pixel tiles at three waves per
+// SIMD: 0.192 ms; 2x2 -- 16-pixel tiles, three 52000-byte workgroups per CU,
+// 0.109 ms, 32-pixel tiles: 0.123 ms).
 template <int KH>
 struct Bf16Tile {
   static constexpr int TW = 32, LDS = 80 * 1024, MINW = 1;
@@ -534,6 +771,23 @@ template <>
 struct Bf16Tile<2> {
   static constexpr int TW = 16, LDS = 52000, MINW = 3;
 };
+
+template <int KH, int KW>
+static bool try_bf16_st(const ConvShape& s, const void* x, const void* wpk, const float* b, int act, void* y,
+                        hipStream_t st) {
+  if (s.KH != KH || s.KW != KW || feature_stride(s.C) != 104 || (s.K + 15) / 16 != 7) return false;
+  using Cfg = StCfg<KH, KW>;
+  const auto kern = k_conv_bf16_st<KH, KW>;
+  const int Ho = s.H - KH + 1, Wo = s.W - KW + 1;
+  static bool configured = false;
+  if (!configured) {
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::lds_bytes);
+    configured = true;
+  }
+  kern<<<dim3(ceil_div(Wo, Cfg::TW), ceil_div(Ho, kTH)), 256, Cfg::lds_bytes, st>>>(
+      (const bf16*)x, s.H, s.W, (const bf16*)wpk, b, s.K, act, (bf16*)y, Ho, Wo);
+  return true;
+}
 
 template <int KH, int KW, int CG, int NT>
 static bool try_bf16(const ConvShape& s, const void* x, const void* wpk, const float* b, int act, void* y,
@@ -586,7 +840,7 @@ static bool mfma_supported(const ConvShape& s) {
 static bool launch_mfma(const ConvShape& s, int precision, const void* x, const void* wpk, const float* b,
                         int act, void* y, hipStream_t st) {
   if (precision == CCS_BF16) {
-    return try_bf16<3, 3, 13, 7>(s, x, wpk, b, act, y, st) || try_bf16<2, 2, 13, 7>(s, x, wpk, b, act, y, st) ||
+    return try_bf16_st<3, 3>(s, x, wpk, b, act, y, st) || try_bf16_st<2, 2>(s, x, wpk, b, act, y, st) ||
            try_bf16<3, 3, 8, 4>(s, x, wpk, b, act, y, st) || try_bf16<3, 3, 1, 1>(s, x, wpk, b, act, y, st) ||
            try_bf16<2, 2, 1, 1>(s, x, wpk, b, act, y, st);
   }
@@ -599,7 +853,7 @@ static size_t packed_bytes(int K, int C, int KH, int KW, int precision) {
   const int Cs = feature_stride(C);
   const int NP = 16 * ((K + 15) / 16);
   if (precision == CCS_BF16) {
-    const int NQ = (KH * KW * Cs / 8 + 3) / 4;
+    const int NQ = packed_chunks(Cs / 8, KH * KW);
     return (size_t)NQ * NP * RPB * 2;
   }
   const int NQ = (KH * KW * Cs + 3) / 4;
@@ -622,7 +876,7 @@ extern "C" int ccs_conv_pack_weights(ccs_handle_t h, const float* w, int K, int 
   const int Cs = feature_stride(C);
   const int NP = 16 * ((K + 15) / 16);
   if (precision == CCS_BF16) {
-    const int NQ = (KH * KW * Cs / 8 + 3) / 4;
+    const int NQ = packed_chunks(Cs / 8, KH * KW);
     const long total = (long)NQ * NP * RPB;
     k_pack_bf16<<<ceil_div(total, 256), 256, 0, h->stream>>>(w, K, C, Cs, KH, KW, NQ, NP, (bf16*)packed);
   } else {
