@@ -25,7 +25,7 @@
 // of large volumes store every seg_len()-th one and recompute the rest), and the
 // per-node minima go to LDS; their float64 sums are formed by wave scans after
 // each pass.  The row decode runs its two message passes on two wavefronts at
-// once (k_decode2).
+// once, and two more waves take its decisions (k_decode4).
 //
 // This file holds the chain sweeps only.  It is compiled with -fno-honor-nans:
 // the min/max trees then run as fused DPP instructions without NaN-quieting
@@ -468,19 +468,29 @@ __device__ double chain_kept(const float* v, const double* b, int n, int lane, d
 // (clamped to the last node, so it never leaves the chain).  The last partial
 // block is an unrolled tail that issues no loads.
 template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK, bool PE>
-__global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 128 : 64) void k_chain(ChainArgs a) {
-  extern __shared__ double b_lds[];  // b_i = offset of the true backward message at node i
+__global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_chain(ChainArgs a) {
+  extern __shared__ double lds_all[];
   constexpr bool kCols = MODE == kColsMinorant;
   // Rows with a message wave: wave 0 runs both passes, wave 1 recomputes the
-  // forward pass's messages into the LDS ring (see kRowRing).
+  // forward pass's messages into the LDS ring (see kRowRing).  A workgroup of
+  // 256 threads runs two chains (waves 0-1 and 2-3, each with its own LDS), so
+  // that between one and two times as many chains as CUs need not put two
+  // workgroups on a CU.
   constexpr bool kRowSeg = !kCols && SEG;
   constexpr int KR = kRowSeg ? seg_len<LPL>() : 1;
   static_assert(!kRowSeg || P == kRowRing * KR, "the forward ring spans the message ring");
   static_assert(P <= 64, "one weight per lane and step of a block");
-  const int c = blockIdx.x;
-  const int lane = threadIdx.x & 63;
-  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  constexpr int kNS = 64 * LPL;  // ring stride per node, floats
   const int n = a.n, D = a.D;
+  const int cs = kRowSeg ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 7) : 0;  // chain slot of the workgroup
+  const int cpw = kRowSeg ? (int)(blockDim.x >> 7) : 1;
+  const int c_slot = blockIdx.x * cpw + cs;
+  const int c = min(c_slot, a.n_chains - 1);  // a missing second chain only keeps the barriers
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & (kRowSeg ? 1 : 63));  // wave in the chain
+  // per chain: b (doubles), w, u, [w2], v, then the ring (floats, 16-byte aligned)
+  const int chain_floats = (((PE ? 6 : 5) * n + 3) & ~3) + (kRowSeg ? kRowRing * KR * kNS : 0);
+  double* b_lds = lds_all + cs * (chain_floats / 2);  // b_i = offset of the true backward message at node i
   ShiftCarry cc;
   const float L1v = in_vgpr(a.L1), L2v = in_vgpr(a.L2);
   const long pbase = (long)c * a.pcs;
@@ -515,14 +525,22 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 128 : 64) void k_cha
   float* v_lds = w2_lds + (PE ? n : 0);
   // (offset arithmetic on the LDS pointer itself keeps its address space)
   float* ring = reinterpret_cast<float*>(b_lds) + (((PE ? 6 : 5) * n + 3) & ~3);
-  constexpr int kNS = 64 * LPL;  // ring stride per node, floats
   float* fw_lds = kRowSeg ? v_lds : u_lds;
   const long wstep = a.pns * a.wmul;
-  for (int i = threadIdx.x; i < n - 1; i += blockDim.x) {
+  constexpr int kChainThreads = kRowSeg ? 128 : 64;
+  for (int i = threadIdx.x & (kChainThreads - 1); i < n - 1; i += kChainThreads) {
     w_lds[i] = wb[(long)i * wstep];
     if constexpr (PE) w2_lds[i] = wb2[(long)i * wstep];
   }
   __syncthreads();
+  if constexpr (kRowSeg) {
+    if (c_slot >= a.n_chains) {  // the workgroup's second chain is missing: the same barriers, no work
+      const int M = (n - 1 + KR - 1) / KR;
+      __syncthreads();
+      for (int k = 0; k < (M == 0 ? 0 : (M > 2 ? M - 1 : 1)); ++k) lds_barrier();
+      return;
+    }
+  }
 
   Vec<LPL> ra[P], rc[P], rb[P];
   // Row chains (the host guarantees a row's byte extent fits in 31 bits): the
@@ -1007,18 +1025,27 @@ __device__ __forceinline__ void ring_pass(Rsrc rA, int strideB, Rsrc rM, const f
     if (s0 + j < cnt) step(first + dir * (s0 + j), ra[j], rm[j], lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f);
 }
 
-// Row decode with both message passes at once: wave 0 runs the forward
-// messages F from node 0, wave 1 the backward messages Bh from node n-1.  Until
-// the midpoint each stores its messages; after one workgroup barrier each
-// decodes the nodes on the far side, where the other wave's messages already
-// sit in the buffer (wave 0 the right half, wave 1 the left half).  F_i + Bh_i
-// and the decision rule are those of the single-wave decode, so the labels are
-// the same; the chain latency is n steps instead of 2n.
+// Row decode on four waves.  Wave 0 runs the forward messages F from node 0,
+// wave 1 the backward messages Bh from node n-1.  Until the midpoint each
+// stores its messages in the scratch volume (phase 1); then each goes on into
+// the other half (phase 2) and hands its messages to a decision wave through
+// an LDS ring of two segments of K nodes: wave 2 decides the nodes mid .. n-1
+// (F_i from wave 0's ring, Bh_i from the scratch volume), wave 3 the nodes
+// mid-1 .. 0 (Bh_i from wave 1's ring, F_i from the scratch volume).  Phase 2
+// runs in rounds: in round r the message waves write segment r into slot r % 2
+// and the decision waves decide segment r - 1 from slot (r - 1) % 2; all four
+// waves meet at a barrier after every round.  F_i + Bh_i and the decision rule
+// are those of the single-wave decode, so the labels are the same; the chain
+// latency is n message steps, with no decision work on the message waves.
 template <int LPL, int P, bool FULL, int DC, bool GAP, bool PE>
-__global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
+__global__ __launch_bounds__(256) void k_decode4(ChainArgs a) {
   extern __shared__ double lds2[];
+  constexpr int K = seg_len<LPL>();
+  static_assert(P == kRowRing * K, "the message waves' register ring spans three segments");
+  static_assert(P <= 64, "one weight per lane and step of a block");
+  constexpr int kNS = 64 * LPL;  // ring stride per node, floats
   const int c = blockIdx.x;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform branch
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform branches
   const int lane = threadIdx.x & 63;
   const int n = a.n, D = a.D;
   ShiftCarry cc;
@@ -1032,6 +1059,7 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   float* w_lds = reinterpret_cast<float*>(lds2);
   float* u_lds = w_lds + n;
   float* w2_lds = u_lds + n;  // PE only
+  float* ring = w_lds + (((PE ? 3 : 2) * n + 3) & ~3);  // [half][slot][K][kNS]
   const int strideB = (int)(nstride * 4);
   const Rsrc rA = chain_rsrc(Ab, (long)(n - 1) * nstride + D);
   const Rsrc rM = chain_rsrc(msg, (long)n * D);
@@ -1039,12 +1067,14 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
 #pragma unroll
   for (int j = 0; j < LPL; ++j) valid[j] = FULL || lane * LPL + j < D;
   const long wstep = a.pns * a.wmul;
-  for (int i = threadIdx.x; i < n - 1; i += 128) {
+  for (int i = threadIdx.x; i < n - 1; i += 256) {
     w_lds[i] = wb[(long)i * wstep];
     if constexpr (PE) w2_lds[i] = wb2[(long)i * wstep];
   }
   __syncthreads();
   const int mid = n / 2;
+  const int len0 = n - mid, len1 = mid;  // phase-2 nodes of waves 0 / 1
+  const int S = ((len0 > len1 ? len0 : len1) + K - 1) / K;  // rounds with a segment
 
   Vec<LPL> X;  // wave 0: F_i, wave 1: Bh_i
   auto fwd = [&](const Vec<LPL>& an, float w, float w2) {  // F_i = A_i + minconv(F_{i-1}; w_{i-1}) - min F_{i-1}
@@ -1065,8 +1095,7 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
   };
   // Label (and, with GAP, the min-marginal gap) of node i from F_i + Bh_i.
   // Every lane stores the same value to the same address (one write, no
-  // branch), so a decision has no control flow and its instructions can
-  // interleave with the next node's message update.
+  // branch).
   auto decide = [&](int i, const Vec<LPL>& F, const Vec<LPL>& b) {
     const long p = pbase + (long)i * a.pns;
     float best = kInfF;
@@ -1093,77 +1122,149 @@ __global__ __launch_bounds__(128) void k_decode2(ChainArgs a) {
     a.labels[p] = label;
   };
 
-  // Phase 1: each wave walks to the midpoint, storing its messages.
+  // Phase 1: waves 0 and 1 walk to the midpoint, storing their messages.
   if (wv == 0) {
     X = ld_node<LPL, FULL>(Ab, lane, D, kInfF);  // F_0 = A_0
     if (mid > 0) st_node<LPL, FULL>(msg, lane, D, X);
     ring_pass<LPL, FULL, P, false, PE>(rA, strideB, rM, w_lds, w2_lds, n, D, lane, 1, mid - 1, 1, 0, -1,
                                        [&](int i, const Vec<LPL>& an, const Vec<LPL>&, float w, float w2) {
                                          fwd(an, w, w2);
-                                     st_buf<LPL, FULL>(rM, i * D * 4, lane, D, X);
-                                   });
-  } else {
+                                         st_buf<LPL, FULL>(rM, i * D * 4, lane, D, X);
+                                       });
+  } else if (wv == 1) {
 #pragma unroll
     for (int k = 0; k < LPL; ++k) X.v[k] = 0.f;  // Bh_{n-1} = 0
     st_node<LPL, FULL>(msg + (long)(n - 1) * D, lane, D, X);
     ring_pass<LPL, FULL, P, false, PE>(rA, strideB, rM, w_lds, w2_lds, n, D, lane, n - 2, n - 1 - mid, -1, 1, 0,
                                        [&](int i, const Vec<LPL>& an1, const Vec<LPL>&, float w, float w2) {
                                          bwd(i, an1, w, w2);
-                                     st_buf<LPL, FULL>(rM, i * D * 4, lane, D, X);
-                                   });
+                                         st_buf<LPL, FULL>(rM, i * D * 4, lane, D, X);
+                                       });
   }
   __threadfence_block();
-  __syncthreads();  // executed by both waves: the other half's messages are now visible
-  // Phase 2: each wave continues into the other half and decodes it.  The
-  // decision for a node is taken one step later, beside the next node's
-  // message update (two independent dependency chains in one step); the
-  // first node of the half is handled before the ring, the last after it.
-  Vec<LPL> Fp, Bp;  // min-marginal terms of the node whose decision is pending
-  if (wv == 0) {     // nodes mid .. n-1, forward messages F
-    if (mid > 0) fwd(ld_buf<LPL, FULL>(rA, mid * strideB, lane, D, kInfF), w_lds[mid - 1], PE ? w2_lds[mid - 1] : 0.f);
-    Fp = X;
-    Bp = ld_buf<LPL, FULL>(rM, mid * D * 4, lane, D, 0.f);
-    ring_pass<LPL, FULL, P, true, PE>(rA, strideB, rM, w_lds, w2_lds, n, D, lane, mid + 1, n - mid - 1, 1, 0, -1,
-                                      [&](int i, const Vec<LPL>& an, const Vec<LPL>& bh, float w, float w2) {
-                                        decide(i - 1, Fp, Bp);
-                                        fwd(an, w, w2);
-                                    Fp = X;
-                                    Bp = bh;
-                                  });
-    decide(n - 1, Fp, Bp);
-  } else if (mid > 0) {  // nodes mid-1 .. 0, backward messages Bh
-    bwd(mid - 1, ld_buf<LPL, FULL>(rA, mid * strideB, lane, D, kInfF), w_lds[mid - 1], PE ? w2_lds[mid - 1] : 0.f);
-    Fp = ld_buf<LPL, FULL>(rM, (mid - 1) * D * 4, lane, D, 0.f);
-    Bp = X;
-    ring_pass<LPL, FULL, P, true, PE>(rA, strideB, rM, w_lds, w2_lds, n, D, lane, mid - 2, mid - 1, -1, 1, 0,
-                                      [&](int i, const Vec<LPL>& an1, const Vec<LPL>& F, float w, float w2) {
-                                        decide(i + 1, Fp, Bp);
-                                        bwd(i, an1, w, w2);
-                                    Fp = F;
-                                    Bp = X;
-                                  });
-    decide(0, Fp, Bp);
-  }
-  if (wv == 1) {
-    // Chain minimum: sum of the backward minima + min(A_0 + Bh_0).
-    const Vec<LPL> a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
-    Vec<LPL> M0;
+  __syncthreads();  // the phase-1 messages are visible to the decision waves
+
+  if (wv < 2) {
+    // Phase 2, message waves.  Step j: wave 0 forms F at node mid + j (from
+    // A_{mid+j} and w_{mid+j-1}; F_0 = A_0 is X already when mid = 0), wave 1
+    // Bh at node mid-1-j (from A_{mid-j} and w_{mid-1-j}).  A runs P steps ahead
+    // in a register ring; the block's weights and refill offsets sit one per
+    // lane (v_readlane per step).
+    const int len = wv == 0 ? len0 : len1;
+    float* rbase = ring + wv * 2 * K * kNS;
+    auto anode = [&](int j) { return wv == 0 ? min(mid + j, n - 1) : max(mid - j, 0); };
+    auto wnode = [&](int j) { return max(min(wv == 0 ? mid + j - 1 : mid - 1 - j, n - 2), 0); };
+    Vec<LPL> ra[P];
 #pragma unroll
-    for (int k = 0; k < LPL; ++k) M0.v[k] = a0.v[k] + X.v[k];
-    const double Lambda = chain_total(u_lds, n, lane) + (double)wave_min(local_min(M0));
-    if (lane == 0) a.lam[c] = Lambda;
+    for (int t = 0; t < P; ++t) ra[t] = ld_buf<LPL, FULL>(rA, anode(t) * strideB, lane, D, kInfF);
+    for (int jb = 0; jb < S * K; jb += P) {
+      const float wl = w_lds[wnode(jb + lane)];
+      const float wl2 = PE ? w2_lds[wnode(jb + lane)] : 0.f;
+      const int ao = anode(jb + P + lane) * strideB;
+      const int r0 = jb / K;
+#pragma unroll
+      for (int t = 0; t < P; ++t) {
+        const int j = jb + t;
+        const Vec<LPL> an = fresh(ra[t]);
+        __builtin_amdgcn_sched_barrier(kSchedValu);
+        ra[t] = ld_buf<LPL, FULL>(rA, __builtin_amdgcn_readlane(ao, t), lane, D, kInfF);
+        if (j < len) {
+          if (wv == 0) {
+            if (j > 0 || mid > 0) fwd(an, lane_weight(wl, t), PE ? lane_weight(wl2, t) : 0.f);
+          } else {
+            bwd(mid - 1 - j, an, lane_weight(wl, t), PE ? lane_weight(wl2, t) : 0.f);
+          }
+          st_ring<LPL>(rbase + (((r0 + t / K) & 1) * K + t % K) * kNS, lane, X);
+        }
+        if ((t + 1) % K == 0 && r0 + t / K < S) lds_barrier();  // end of round r0 + t / K
+      }
+    }
+    lds_barrier();  // round S (the decision waves' last segment)
+    if (wv == 1) {
+      // Chain minimum: sum of the backward minima + min(A_0 + Bh_0).
+      const Vec<LPL> a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
+      Vec<LPL> M0;
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) M0.v[k] = a0.v[k] + X.v[k];
+      const double Lambda = chain_total(u_lds, n, lane) + (double)wave_min(local_min(M0));
+      if (lane == 0) a.lam[c] = Lambda;
+    }
+  } else {
+    // Phase 2, decision waves: wave 2 takes node mid + j with F from wave 0's
+    // ring and Bh from the scratch volume, wave 3 node mid-1-j with Bh from
+    // wave 1's ring and F from the scratch volume.  The scratch messages of the
+    // next segment are loaded while this one is decided.
+    const int h = wv - 2;
+    const int len = h == 0 ? len0 : len1;
+    const float* rbase = ring + h * 2 * K * kNS;
+    auto node_of = [&](int j) { return h == 0 ? mid + j : mid - 1 - j; };
+    auto load_other = [&](int s, Vec<LPL> (&o)[K]) {
+#pragma unroll
+      for (int t = 0; t < K; ++t) {
+        const int i = min(max(node_of(s * K + t), 0), n - 1);
+        o[t] = ld_buf<LPL, FULL>(rM, i * D * 4, lane, D, 0.f);
+      }
+    };
+    auto decide_seg = [&](int s, const Vec<LPL> (&o)[K]) {
+      const float* slot = rbase + (s & 1) * K * kNS;
+#pragma unroll
+      for (int t = 0; t < K; ++t) {
+        const int j = s * K + t;
+        if (j < len) {
+          const Vec<LPL> mine = ld_ring<LPL>(slot + t * kNS, lane);
+          if (h == 0)
+            decide(node_of(j), mine, o[t]);
+          else
+            decide(node_of(j), o[t], mine);
+        }
+      }
+    };
+    Vec<LPL> OX[K], OY[K];
+    load_other(0, OX);
+    lds_barrier();  // round 0: segment 0 is written
+    for (int r = 1; r <= S;) {
+      load_other(min(r, S - 1), OY);
+      decide_seg(r - 1, OX);
+      lds_barrier();
+      if (++r > S) break;
+      load_other(min(r, S - 1), OX);
+      decide_seg(r - 1, OY);
+      lds_barrier();
+      ++r;
+    }
   }
+}
+
+// Compute units of the current device (cached per device).
+static int cu_count() {
+  static int cus[64] = {0};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
 }
 
 template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK, bool PE>
 static void launch_sweep(ChainArgs& a, size_t lds, hipStream_t st) {
   constexpr bool kRowSeg = MODE != kColsMinorant && SEG;
-  if constexpr (kRowSeg)  // v_lds, the alignment of the ring, and the ring
-    lds += sizeof(float) * (size_t)a.n + 16 + sizeof(float) * kRowRing * seg_len<LPL>() * 64 * LPL;
+  int cpw = 1;
+  if constexpr (kRowSeg) {
+    // per chain: b, w, u, [w2], v floats rounded to 16 bytes, then the ring
+    lds = sizeof(float) * ((((PE ? 6 : 5) * (size_t)a.n + 3) & ~(size_t)3) +
+                           (size_t)kRowRing * seg_len<LPL>() * 64 * LPL);
+    const int cus = cu_count();
+    if (a.n_chains > cus && a.n_chains <= 2 * cus) cpw = 2;  // one workgroup per CU
+    lds *= cpw;
+  }
   if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
     (void)hipFuncSetAttribute((const void*)k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK, PE>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK, PE><<<a.n_chains, kRowSeg ? 128 : 64, lds, st>>>(a);
+  k_chain<LPL, MODE, P, FULL, DC, SEG, NANCHK, PE><<<(a.n_chains + cpw - 1) / cpw, kRowSeg ? 128 * cpw : 64, lds, st>>>(a);
 }
 
 // Register-ring depth of the row sweeps with a message wave: three segments.
@@ -1175,11 +1276,20 @@ constexpr int row_seg_p() {
 template <int LPL, int MODE, int P, bool FULL, int DC, bool PE>
 static void launch_one(ChainArgs& a, size_t lds, bool seg, hipStream_t st) {
   if constexpr (MODE == kRowsDecode) {
-    const size_t lds2 = (PE ? 3 : 2) * sizeof(float) * (size_t)a.n;
+    // w (and w2), u, then the two halves' two-segment message rings at a 16-byte boundary
+    const size_t lds2 = sizeof(float) * ((((PE ? 3 : 2) * (size_t)a.n + 3) & ~(size_t)3) +
+                                         (size_t)2 * 2 * seg_len<LPL>() * 64 * LPL);
+    constexpr int PD = row_seg_p<LPL>();
+    if (lds2 > 64 * 1024) {
+      (void)hipFuncSetAttribute((const void*)k_decode4<LPL, PD, FULL, DC, true, PE>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
+      (void)hipFuncSetAttribute((const void*)k_decode4<LPL, PD, FULL, DC, false, PE>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
+    }
     if (a.gap)
-      k_decode2<LPL, P, FULL, DC, true, PE><<<a.n_chains, 128, lds2, st>>>(a);
+      k_decode4<LPL, PD, FULL, DC, true, PE><<<a.n_chains, 256, lds2, st>>>(a);
     else
-      k_decode2<LPL, P, FULL, DC, false, PE><<<a.n_chains, 128, lds2, st>>>(a);
+      k_decode4<LPL, PD, FULL, DC, false, PE><<<a.n_chains, 256, lds2, st>>>(a);
   } else if constexpr (MODE == kColsMinorant) {
     if (seg)
       launch_sweep<LPL, MODE, P, FULL, DC, true, false, PE>(a, lds, st);
