@@ -12,6 +12,7 @@
 // accumulates in float32).  Reductions over pixels are split into blocks and
 // summed in a fixed order: results do not depend on scheduling.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -182,6 +183,140 @@ __global__ __launch_bounds__(64 * KT, 4) void k_dw_mfma(const TX* __restrict__ x
     for (int e = 0; e < 4; ++e) out[(k0 + 4 * lk + e) * CP + 16 * t + lrow] = acc[t][e];
 }
 
+// Weight gradient of one tap row on bf16 MFMA (bf16 input and dz): the block
+// handles the KW taps (i, 0 .. KW-1) of tap row i for a chunk of 64-pixel
+// segments, one wave per 16-channel tile of K (KT waves), KW x CT accumulator
+// tiles per wave.  A segment's dz and its input row go to LDS transposed
+// (channel-major, pixel-minor bf16), the input once per tap shift j (so every
+// fragment is a 16-byte aligned read): per 32 pixels a wave issues KW x CT
+// v_mfma_f32_16x16x32_bf16 with A[m][p] = dz[p][k0+m], B[p][n] = x[p + j][16t+n].
+// Two LDS buffers alternate: while the MFMAs run on one segment, the next one's
+// vectors (loaded a segment earlier) are written into the other.  The K x C
+// sums of tap (i, j) go to part[chunk][tap], the layout of k_dw_mfma.
+template <int KT, int CT, int KW>
+__global__ __launch_bounds__(64 * KT, 1) void k_dw_bf16(const bf16* __restrict__ x, int Win, int Csx,
+                                                        const bf16* __restrict__ dz, int Ho, int Wo, int Csk, int KH,
+                                                        int nch, int segs_per_row, long nseg, long spc,
+                                                        float* __restrict__ part) {
+  constexpr int KP = 16 * KT, CP = 16 * CT, NTH = 64 * KT, SEG = 64, XPX = SEG + KW - 1;
+  constexpr int ZB = KP * SEG, XB = KW * CP * SEG;  // bf16 elements of one buffer's dz and x parts
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  bf16* lds = reinterpret_cast<bf16*>(lds_raw);  // [buf][ZB + XB]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int b = blockIdx.x, xcd = b & 7, local = b >> 3;
+  const int ti = local % KH, chunk = (local / KH) * 8 + xcd;
+  if (chunk >= nch) return;  // the grid is rounded up to whole groups of 8 chunks
+  const long s0 = (long)chunk * spc;
+  const long s1 = s0 + spc < nseg ? s0 + spc : nseg;
+  for (int i = tid; i < 2 * (ZB + XB) / 2; i += NTH) reinterpret_cast<unsigned*>(lds)[i] = 0u;  // pad channels stay 0
+  const int zvp = Csk / 8, xvp = Csx / 8;  // 16-byte vectors per pixel
+  const int nzv = SEG * zvp, nv = nzv + XPX * xvp;
+  constexpr int NVMAX = (SEG * (KP / 8) + XPX * (CP / 8) + NTH - 1) / NTH;
+  // Per vector slot: pixel, channel group, operand (0 dz, 1 x, -1 none).
+  // Consecutive slots take consecutive pixels of one channel group, so the
+  // transposed 16-bit stores of a wave fill contiguous 128-byte row pieces (no
+  // bank conflicts); the group's strided global reads hit the L1 lines that the
+  // segment's other groups bring in.
+  int vpx[NVMAX], vg[NVMAX], vkind[NVMAX];
+#pragma unroll
+  for (int i = 0; i < NVMAX; ++i) {
+    const int v = tid + i * NTH;
+    if (v < nzv) {
+      vpx[i] = v % SEG; vg[i] = v / SEG; vkind[i] = 0;
+    } else if (v < nv) {
+      vpx[i] = (v - nzv) % XPX; vg[i] = (v - nzv) / XPX; vkind[i] = 1;
+    } else {
+      vpx[i] = vg[i] = 0; vkind[i] = -1;
+    }
+  }
+  u32x4 reg[NVMAX];
+  auto load = [&](long s) {
+    const int ho = (int)(s / segs_per_row), wo0 = (int)(s % segs_per_row) * SEG;
+    const bf16* zrow = dz + ((long)ho * Wo + wo0) * Csk;
+    const bf16* xrow = x + ((long)(ho + ti) * Win + wo0) * Csx;
+#pragma unroll
+    for (int i = 0; i < NVMAX; ++i) {
+      u32x4 r = {0u, 0u, 0u, 0u};
+      if (vkind[i] == 0) {
+        if (wo0 + vpx[i] < Wo) r = *reinterpret_cast<const u32x4*>(zrow + (long)vpx[i] * Csk + vg[i] * 8);
+      } else if (vkind[i] == 1) {
+        if (wo0 + vpx[i] < Win) r = *reinterpret_cast<const u32x4*>(xrow + (long)vpx[i] * Csx + vg[i] * 8);
+      }
+      reg[i] = r;
+    }
+  };
+  // Transposed stores: element e of a vector (channel 8 g + e) goes to row
+  // 8 g + e at the vector's pixel; an input vector at pixel p' goes to shift
+  // copy j at pixel p' - j.
+  auto store = [&](int buf) {
+    bf16* zb = lds + buf * (ZB + XB);
+    bf16* xb = zb + ZB;
+#pragma unroll
+    for (int i = 0; i < NVMAX; ++i) {
+      const unsigned short* e16 = reinterpret_cast<const unsigned short*>(&reg[i]);
+      if (vkind[i] == 0) {
+        unsigned short* d = reinterpret_cast<unsigned short*>(zb) + (vg[i] * 8) * SEG + vpx[i];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) d[e * SEG] = e16[e];
+      } else if (vkind[i] == 1) {
+#pragma unroll
+        for (int j = 0; j < KW; ++j) {
+          const int p = vpx[i] - j;
+          if (p >= 0 && p < SEG) {
+            unsigned short* d = reinterpret_cast<unsigned short*>(xb) + (j * CP + vg[i] * 8) * SEG + p;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) d[e * SEG] = e16[e];
+          }
+        }
+      }
+    }
+  };
+  f32x4 acc[KW][CT];
+#pragma unroll
+  for (int j = 0; j < KW; ++j)
+#pragma unroll
+    for (int t = 0; t < CT; ++t) acc[j][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int lrow = lane & 15, kg = lane >> 4, k0 = 16 * wave;
+  __syncthreads();  // the zero fill is done
+  if (s0 < s1) {
+    load(s0);
+    store(0);
+    if (s0 + 1 < s1) load(s0 + 1);
+  }
+  __syncthreads();
+  for (long s = s0; s < s1; ++s) {
+    const int buf = (int)((s - s0) & 1);
+    const bf16* zb = lds + buf * (ZB + XB);
+    const bf16* xb = zb + ZB;
+    // the next segment into the other buffer (its previous readers finished
+    // before the barrier that ended the last iteration)
+    if (s + 1 < s1) {
+      store(buf ^ 1);
+      if (s + 2 < s1) load(s + 2);
+    }
+#pragma unroll
+    for (int h = 0; h < SEG / 32; ++h) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(zb + (k0 + lrow) * SEG + 32 * h + 8 * kg);
+#pragma unroll
+      for (int j = 0; j < KW; ++j)
+#pragma unroll
+        for (int t = 0; t < CT; ++t) {
+          const bf16x8 bv = *reinterpret_cast<const bf16x8*>(xb + (j * CP + 16 * t + lrow) * SEG + 32 * h + 8 * kg);
+          acc[j][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[j][t], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < KW; ++j) {
+    float* out = part + ((long)chunk * KH * KW + ti * KW + j) * KP * CP;
+#pragma unroll
+    for (int t = 0; t < CT; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) out[(k0 + 4 * kg + e) * CP + 16 * t + lrow] = acc[j][t][e];
+  }
+}
+
 // dW[k][c][KH-1-i][KW-1-j] = sum over chunks of the (tap = i KW + j) partials.
 __global__ void k_dw_reduce(const float* __restrict__ part, int nch, int taps, int KP, int CP, int K, int C, int KH,
                             int KW, float* __restrict__ dw) {
@@ -291,8 +426,29 @@ bool dw_mfma(int KT, int CT, const TX* x, int W, int Csx, const T* z, int Ho, in
   return false;
 }
 
-constexpr int kDwSeg = 64;            // output pixels per segment of k_dw_mfma
+constexpr int kDwSeg = 64;            // output pixels per segment of k_dw_mfma and k_dw_bf16
 constexpr int kDwTargetBlocks = 512;  // two resident blocks per CU of the 256: one round
+constexpr int kDwBf16Blocks = 256;    // k_dw_bf16: one 114 KB block per CU
+
+void launch_dw_bf16(const bf16* x, int W, int Csx, const bf16* z, int Ho, int Wo, int Csk, int KH, int KW,
+                    int segs_row, long nseg, long spc, int nch, float* part, hipStream_t st) {
+  const unsigned blocks = (unsigned)(8 * KH * ((nch + 7) / 8));  // whole groups of 8 chunks, one per XCD
+  auto go = [&](auto kwc) {
+    constexpr int kw = decltype(kwc)::value;
+    constexpr size_t lds = 2 * sizeof(bf16) * (112 * 64 + kw * 112 * 64);
+    const auto kern = k_dw_bf16<7, 7, kw>;
+    static bool configured = false;
+    if (!configured) {
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      configured = true;
+    }
+    kern<<<blocks, 64 * 7, lds, st>>>(x, W, Csx, z, Ho, Wo, Csk, KH, nch, segs_row, nseg, spc, part);
+  };
+  if (KW == 3)
+    go(std::integral_constant<int, 3>{});
+  else
+    go(std::integral_constant<int, 2>{});
+}
 
 // The backward of one layer from its input x (element type TX) and output y
 // (element type T of the precision); dy has y's shape and type, dx x's shape
@@ -311,7 +467,12 @@ int conv_backward(ccs_handle_t h, const TX* x, const T* y, int H, int W, int C, 
   const bool mfma = KT && CT && Csk <= 16 * KT && Csx <= 16 * CT;
   const int segs_row = (Wo + kDwSeg - 1) / kDwSeg;
   const long nseg = (long)Ho * segs_row;
-  int nch = (int)std::min<long>(nseg, std::max(8, kDwTargetBlocks / taps / 8 * 8));
+  // bf16 input and dz with 7 x 7 channel tiles: the tap-row kernel on bf16 MFMA
+  // (one block per tap row and chunk, about one block per CU in all).
+  const bool dw_bf16 = std::is_same<TX, bf16>::value && std::is_same<T, bf16>::value && mfma && KT == 7 &&
+                       CT == 7 && (KW == 2 || KW == 3);
+  const int dw_units = dw_bf16 ? KH : taps;  // blocks per chunk
+  int nch = (int)std::min<long>(nseg, std::max(8, (dw_bf16 ? kDwBf16Blocks : kDwTargetBlocks) / dw_units / 8 * 8));
   const long spc = (nseg + nch - 1) / nch;
   nch = (int)((nseg + spc - 1) / spc);
   const int nb = 64;  // pixel blocks of the VALU weight gradient
@@ -337,7 +498,12 @@ int conv_backward(ccs_handle_t h, const TX* x, const T* y, int H, int W, int C, 
     k_reduce_cols<<<Csk, 256, 0, h->stream>>>(part, nbz, part + (size_t)nbz * Csk);
     CCS_HIP(hipMemcpyAsync(db, part + (size_t)nbz * Csk, sizeof(float) * K, hipMemcpyDeviceToDevice, h->stream));
   }
-  if (mfma) {
+  if (dw_bf16) {
+    if constexpr (std::is_same<TX, bf16>::value && std::is_same<T, bf16>::value)
+      launch_dw_bf16(x, W, Csx, z, Ho, Wo, Csk, KH, KW, segs_row, nseg, spc, nch, part, h->stream);
+    k_dw_reduce<<<ceil_div((long)wcount, 256), 256, 0, h->stream>>>(part, nch, taps, 16 * KT, 16 * CT, K, C, KH, KW,
+                                                                  dw);
+  } else if (mfma) {
     dw_mfma<TX, T>(KT, CT, x, W, Csx, z, Ho, Wo, Csk, KW, taps, segs_row, nseg, spc, nch, part, h->stream);
     k_dw_reduce<<<ceil_div((long)wcount, 256), 256, 0, h->stream>>>(part, nch, taps, 16 * KT, 16 * CT, K, C, KH, KW,
                                                                   dw);

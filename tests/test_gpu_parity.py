@@ -184,6 +184,41 @@ def test_conv_backward_stored_bf16(handle, C, K, k, x_bf16):
         assert rel(from_hwc(dx, C), rdx) < 1e-2
 
 
+@pytest.mark.parametrize("k,H,W", [(3, 9, 70), (3, 21, 150), (2, 16, 131), (3, 66, 300), (2, 5, 40)])
+def test_dw_bf16_kernel_matches_fp32_mfma(handle, k, H, W):
+    """The bf16 weight-gradient kernel (bf16 input and dz, tap rows on
+    v_mfma_f32_16x16x32_bf16) against the float32-MFMA kernel on the same values
+    (the input given as float32): the products are exact in float32, so dW and
+    db agree to the summation order (1e-5 relative, per tap and channel pair
+    within 1e-5 of the largest entry).  Widths that end inside a 64-pixel
+    segment and heights of one to a few segments per chunk are included."""
+    rng = np.random.default_rng(70 + k + H + W)
+    C = K = 100
+    Ho, Wo = H - k + 1, W - k + 1
+    cs = capi.feature_stride(C)
+    x = bf16_round(rng.uniform(-1, 1, size=(C, H, W)).astype(np.float32))
+    y = bf16_round(rng.uniform(-0.95, 0.95, size=(K, Ho, Wo)).astype(np.float32))
+    dy = bf16_round(rng.normal(size=(K, Ho, Wo)).astype(np.float32))
+    w = glorot(rng, K, C, k, k)
+    wd = torch.as_tensor(w, device="cuda")
+    xb = to_hwc(x, cs).to(torch.bfloat16)
+    xf = to_hwc(x, cs)
+    yb = to_hwc(y, cs).to(torch.bfloat16)
+    dyb = to_hwc(dy, cs).to(torch.bfloat16)
+    out = []
+    for xin, xprec in ((xb, capi.BF16), (xf, capi.FP32)):
+        dw = torch.empty((K, C, k, k), device="cuda")
+        db = torch.empty(K, device="cuda")
+        call("ccs_conv2d_backward_stored", handle.h, ptr(xin), xprec, ptr(yb), capi.BF16, H, W, C, ptr(wd), K, k, k,
+             1, ptr(dyb), None, ptr(dw), ptr(db))
+        out.append((dw.cpu().numpy().astype(np.float64), db.cpu().numpy().astype(np.float64)))
+    (dw_b, db_b), (dw_f, db_f) = out
+    scale = np.abs(dw_f).max()
+    assert np.linalg.norm(dw_b - dw_f) <= 1e-5 * np.linalg.norm(dw_f)
+    assert np.abs(dw_b - dw_f).max() <= 1e-5 * scale
+    np.testing.assert_array_equal(db_b, db_f)  # the same bias kernel
+
+
 # ---------------------------------------------------------------- correlation
 
 def offsets(dmin, dstep, D):
