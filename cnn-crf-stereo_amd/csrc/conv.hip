@@ -453,7 +453,12 @@ __global__ __launch_bounds__(256, 2) void k_conv_bf16_st(const bf16* __restrict_
 #pragma unroll
       for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[m], acc[m][t], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
-    wait_vmcnt<2>();  // this wave's loads of chunk q+2 are done
+    // This wave's loads of chunk q+2 are done: only chunk q+3's two loads may
+    // be younger, and there are none when q+3 is past the last chunk.
+    if (q + 3 < Cfg::NQ)
+      wait_vmcnt<2>();
+    else
+      wait_vmcnt<0>();
     tile_barrier();
     load_chunk(q + Cfg::NSLOT);
   };
