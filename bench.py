@@ -130,7 +130,7 @@ def run_frame(ops, h, frame, left, right, L, A, side=None, record=True):
     labels = torch.empty((H, W), dtype=torch.int32, device="cuda")
     for it in range(frame.iters):
         # The first row sweep reads its unaries from the cost volume (direction 3).
-        L.run("crf_rows", {"bytes": 2 * V},
+        L.run("crf_rows", {"bytes": 3 * V},
               lambda it=it: ops.slackprop_sweep(h, cost, A, wx, wy, frame.L1, frame.L2, frame.delta,
                                                 3 if it == 0 else 0))
         L.run("crf_cols", {"bytes": 3 * V}, lambda: ops.slackprop_sweep(h, cost, A, wx, wy, frame.L1, frame.L2,

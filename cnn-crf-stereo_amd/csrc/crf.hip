@@ -217,8 +217,8 @@ static int sweep(ccs_handle_t h, const float* cost, float* A, const float* wx, c
   if (rc) return rc;
   CCS_CHECK(direction >= 0 && direction <= 3, CCS_ERR_ARG, "ccs_slackprop_sweep: direction must be 0, 1, 2 or 3");
   CCS_CHECK(pen || (direction != 0 && direction != 3) || wx, CCS_ERR_ARG, "ccs_slackprop_sweep: rows need wx");
-  CCS_CHECK(direction != 3 || (cost && cost != A), CCS_ERR_ARG,
-            "ccs_slackprop_sweep: the first row sweep reads cost, which must not be A");
+  CCS_CHECK(direction == 2 || (cost && cost != A), CCS_ERR_ARG,
+            "ccs_slackprop_sweep: row and column sweeps read cost, which must not be A");
   CCS_CHECK(direction != 1 || ((pen || wy) && cost), CCS_ERR_ARG, "ccs_slackprop_sweep: columns need cost and wy");
   CCS_CHECK(direction != 2 || ((pen || wx) && labels), CCS_ERR_ARG, "ccs_slackprop_sweep: decode needs wx and labels");
   const size_t vol = (size_t)H * W * D;

@@ -18,6 +18,12 @@
 // on.  The dual bound Phi_rows(A) + Phi_cols(C - A) never decreases; its float64
 // bookkeeping (chain minima S, kept constants K) runs beside the float32 min-sum.
 //
+// The sweeps keep one state volume that alternates between the two halves'
+// unaries: a row sweep reads A and stores B = C - (A - g), a column sweep reads
+// B and stores A = (C - B) + g, so each reads one volume in each of its two
+// passes and the cost volume only in its forward pass (4 volume passes per
+// sweep instead of 3 for rows and 5 for columns).
+//
 // GPU mapping: one wavefront per chain, label s = lane * LPL + j held in VGPRs,
 // DPP wave_shr/wave_shl for the label neighbours and a DPP tree for the minimum
 // over labels.  The volume streams through HBM once per pass (node i is D
@@ -495,11 +501,16 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
   const float L1v = in_vgpr(a.L1), L2v = in_vgpr(a.L2);
   const long pbase = (long)c * a.pcs;
   const long nstride = a.pns * (long)D;  // floats between consecutive nodes
-  // Rows read the unaries from A_in (A, or the cost volume in a run's first
-  // sweep) and write A; columns read and write A.
+  // The running state alternates between the two subproblems' unaries, so
+  // that each sweep reads one volume for its chain unaries:
+  //   rows read A from A_in (A, or the cost volume in a run's first sweep) and
+  //     write the column unaries C - A' (A' = A - g);
+  //   columns read B = C - A from A and write the row unaries A' = (C - B) + g.
+  // Both convert with the cost volume in their forward pass only.  The decode
+  // reads A.
   const float* __restrict__ Ab = (kCols ? a.A : a.A_in) + pbase * D;
   float* __restrict__ Aw = a.A + pbase * D;
-  const float* __restrict__ Cb = kCols ? a.cost + pbase * D : nullptr;
+  const float* __restrict__ Cb = MODE != kRowsDecode ? a.cost + pbase * D : nullptr;
   const float* __restrict__ wb = a.w + pbase * a.wmul;
   const float* __restrict__ wb2 = PE ? a.w2 + pbase * a.wmul : nullptr;
   float* __restrict__ msg = a.msg + (long)c * n * D;
@@ -507,11 +518,26 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
 #pragma unroll
   for (int j = 0; j < LPL; ++j) valid[j] = FULL || lane * LPL + j < D;
 
-  auto theta_of = [&](const Vec<LPL>& av, const Vec<LPL>& cv) {
+  // The chain unaries: what the sweep reads (A for rows, B for columns).
+  auto theta_of = [&](const Vec<LPL>& av) {
     Vec<LPL> th;
 #pragma unroll
-    for (int j = 0; j < LPL; ++j) th.v[j] = valid[j] ? (kCols ? cv.v[j] - av.v[j] : av.v[j]) : kInfF;
+    for (int j = 0; j < LPL; ++j) th.v[j] = valid[j] ? av.v[j] : kInfF;
     return th;
+  };
+  // Columns: the row unaries C - B of a node.
+  auto row_unaries = [&](const Vec<LPL>& bv, const Vec<LPL>& cv) {
+    Vec<LPL> r;
+#pragma unroll
+    for (int j = 0; j < LPL; ++j) r.v[j] = cv.v[j] - bv.v[j];
+    return r;
+  };
+  // Rows: the column unaries C - A' stored for a node.
+  auto col_unaries = [&](const Vec<LPL>& out, const Vec<LPL>& cv) {
+    Vec<LPL> r;
+#pragma unroll
+    for (int j = 0; j < LPL; ++j) r.v[j] = cv.v[j] - out.v[j];
+    return r;
   };
 
   // The chain's edge weights (per-edge P1 and P2 with PE) go to LDS once (the
@@ -598,7 +624,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
 #pragma unroll
         for (int t = K - 1; t >= 0; --t) {
           if (ib + t >= 0) {
-            const Vec<LPL> U = plus(theta_of(B.a[t], B.a[t]), Xb);
+            const Vec<LPL> U = plus(theta_of(B.a[t]), Xb);
             const float u = lane_weight(B.u, t);
             const Vec<LPL> m = minconv<LPL, DC, PE>(U, u, lane_weight(B.w, t), PE ? lane_weight(B.w2, t) : 0.f,
                                                     L1v, L2v, a.delta, cc);
@@ -629,14 +655,12 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
     }
   };
   if (!kRowSeg || wid == 0) {
-  auto load_b = [&](int s, Vec<LPL>& av, Vec<LPL>& cv) {
+  auto load_b = [&](int s, Vec<LPL>& av) {
     const int node = s < nb ? n - 1 - s : 1;  // clamped: beyond the chain re-read node 1
-    if constexpr (kCols) {
+    if constexpr (kCols)
       av = ld_node<LPL, FULL>(Ab + node * nstride, lane, D, kInfF);
-      cv = ld_node<LPL, FULL>(Cb + node * nstride, lane, D, kInfF);
-    } else {
+    else
       av = ld_buf<LPL, FULL>(rA, node * strideB, lane, D, kInfF);
-    }
   };
   // weights of steps s0 .. s0+63 of the backward pass (edge n-2-s)
   auto wlane_b = [&](int s0) { return w_lds[max(n - 2 - (s0 + lane), 0)]; };
@@ -669,7 +693,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
   };
   if (nb > 0) {
 #pragma unroll
-    for (int j = 0; j < P; ++j) load_b(j, ra[j], rc[j]);
+    for (int j = 0; j < P; ++j) load_b(j, ra[j]);
   }
   int s0 = 0;
   float wv = wlane_b(0), wv2 = wlane_b2(0);
@@ -683,12 +707,12 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
       if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
       // Slot j is consumed (U is a new value) before it is refilled, so the
       // refill reuses the slot's registers.
-      const Vec<LPL> U = plus_bh(theta_of(ra[j], rc[j]));
+      const Vec<LPL> U = plus_bh(theta_of(ra[j]));
       __builtin_amdgcn_sched_barrier(kSchedValu);
       if constexpr (kRowSeg)
         ra[j] = ld_buf<LPL, FULL>(rA, __builtin_amdgcn_readlane(bo, j), lane, D, kInfF);
       else
-        load_b(s0 + j + P, ra[j], rc[j]);
+        load_b(s0 + j + P, ra[j]);
       step_b(s0 + j, U, lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f, (j + 1) % KR == 0);
     }
     wv = wnext;
@@ -698,7 +722,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
   for (int j = 0; j < P; ++j)
     if (s0 + j < nb) {
       if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
-      step_b(s0 + j, plus_bh(theta_of(ra[j], rc[j])), lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f,
+      step_b(s0 + j, plus_bh(theta_of(ra[j])), lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f,
              (j + 1) % KR == 0);
     }
   bsum = chain_offsets(u_lds, b_lds, n, lane);  // b_0 = sum of all minima
@@ -710,8 +734,8 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
     const bool any = __ballot(nan_seen) != 0;  // all lanes take part
     if (lane == 0) a.nan_flags[c] = any ? 1 : 0;
   }
-  if constexpr (kCols) c0 = ld_node<LPL, FULL>(Cb, lane, D, kInfF);
-  th0 = theta_of(a0, c0);
+  if constexpr (MODE != kRowsDecode) c0 = ld_node<LPL, FULL>(Cb, lane, D, kInfF);
+  th0 = theta_of(a0);
   }
   __syncthreads();  // kRowSeg: the checkpoints and u_lds are now visible to wave 1
   if constexpr (kRowSeg) {
@@ -727,6 +751,10 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
 
   {
     // Forward pass: keep g_i = gamma (R + Bh) + excess, pass V = R - gamma (R + Bh) on.
+    // Step i's operands: Bh_i, the read unaries of node i+1 and the costs of
+    // node i+1 (columns: they form A_{i+1}) or of node i (rows: they form the
+    // stored C - A'_i).
+    const Rsrc rC = chain_rsrc(Cb, kCols ? 0 : (long)(n - 1) * nstride + D);  // (rows)
     auto load_f = [&](int i, Vec<LPL>& bv, Vec<LPL>& av, Vec<LPL>& cv) {
       const int node = i < nb ? i : nb - 1;
       if constexpr (kCols) {
@@ -736,16 +764,20 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
       } else {
         bv = ld_buf<LPL, FULL>(rM, node * D * 4, lane, D, 0.f);
         av = ld_buf<LPL, FULL>(rA, (node + 1) * strideB, lane, D, kInfF);
+        cv = ld_buf<LPL, FULL>(rC, node * strideB, lane, D, kInfF);
       }
     };
     // weights of steps i0 .. i0+63 of the forward pass (edge i)
     auto wlane_f = [&](int i0) { return w_lds[max(min(i0 + lane, n - 2), 0)]; };
     auto wlane_f2 = [&](int i0) { return PE ? w2_lds[max(min(i0 + lane, n - 2), 0)] : 0.f; };
     Vec<LPL> R = th0;
-    Vec<LPL> a_cur = a0;
+    Vec<LPL> a_cur = kCols ? row_unaries(a0, c0) : a0;  // A_0
+    // Rows: the costs of the last node, for its stored C - A'.
+    Vec<LPL> c_last;
+    if constexpr (!kCols) c_last = ld_buf<LPL, FULL>(rC, (n - 1) * strideB, lane, D, kInfF);
     // The float64 kept constant (the chain minimum this pass leaves behind) is
     // formed from the per-node minima v_i after the pass: chain_kept().
-    // step_f takes RB = R + Bh_i, A_{i+1} and theta_{i+1}.
+    // step_f takes RB = R + Bh_i, A_{i+1} and theta_{i+1} (the read unaries).
     // step_f returns the node's new unaries; the caller stores them.
     auto step_f = [&](int i, const Vec<LPL>& RB, const Vec<LPL>& an, const Vec<LPL>& thn, float wi, float wi2) {
       Vec<LPL> gt, V;
@@ -774,9 +806,10 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
       a_cur = an;
       return out;
     };
-    auto store_f = [&](int i, const Vec<LPL>& out) {
+    // Stores node i's result: columns A'_i, rows C_i - A'_i (ci: C_i).
+    auto store_f = [&](int i, const Vec<LPL>& out, const Vec<LPL>& ci) {
       if constexpr (kCols) st_node<LPL, FULL>(Aw + i * nstride, lane, D, out);
-      else st_buf<LPL, FULL>(rW, i * strideB, lane, D, out);
+      else st_buf<LPL, FULL>(rW, i * strideB, lane, D, col_unaries(out, ci));
     };
     if constexpr (kRowSeg) {
       // Forward step i' = i + g (see row_messages): A_{i+1} in register ring
@@ -790,9 +823,11 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
       const int M = (nb + K - 1) / K;
       const int g = M * K - nb;
       const int nbp = M * K;
-      auto load_a = [&](int ip, Vec<LPL>& av) {
+      // A_{i+1} into ra, C_i into rc (the costs of the node that step stores).
+      auto load_a = [&](int ip, Vec<LPL>& av, Vec<LPL>& cv) {
         const int node = max(min(ip - g, nb - 1), 0) + 1;
         av = ld_buf<LPL, FULL>(rA, node * strideB, lane, D, kInfF);
+        cv = ld_buf<LPL, FULL>(rC, (node - 1) * strideB, lane, D, kInfF);
       };
       // Byte offsets of a block's refills (node of step ip0 + P + lane), one per
       // lane: a step takes its offset with v_readlane instead of clamping it.
@@ -802,7 +837,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
       Vec<LPL> bm[3];
       if (nb > 0) {
 #pragma unroll
-        for (int j = 0; j < P; ++j) load_a(j, ra[j]);
+        for (int j = 0; j < P; ++j) load_a(j, ra[j], rc[j]);
         lds_barrier();  // segments 0 and 1 are in the ring
         bm[0] = ld_ring<LPL>(ring, lane);
         bm[1] = ld_ring<LPL>(ring + kNS, lane);
@@ -819,12 +854,17 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
         }
         const int i = ip - g;
         const Vec<LPL> an = fresh(ra[j]);
+        const Vec<LPL> ci = fresh(rc[j]);
         const Vec<LPL> RB = plus(R, bm[j % 3]);
         __builtin_amdgcn_sched_barrier(kSchedValu);
-        if (!TAIL) ra[j] = ld_buf<LPL, FULL>(rA, __builtin_amdgcn_readlane(ao, j), lane, D, kInfF);
+        if (!TAIL) {
+          const int off = __builtin_amdgcn_readlane(ao, j);
+          ra[j] = ld_buf<LPL, FULL>(rA, off, lane, D, kInfF);
+          rc[j] = ld_buf<LPL, FULL>(rC, off - strideB, lane, D, kInfF);
+        }
         bm[(j + 2) % 3] = ld_ring<LPL>(ring + ((j + 2) % P) * kNS, lane);
         if (!LOW || i >= 0)
-          store_f(i, step_f(i, RB, an, theta_of(an, an), lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f));
+          store_f(i, step_f(i, RB, an, theta_of(an), lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f), ci);
       };
       int i0p = 0;
       if (nbp >= P) {
@@ -890,7 +930,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
 #pragma unroll
         for (int t = K - 1; t >= 0; --t) {
           if (t < cnt) {
-            const Vec<LPL> U = plus(theta_of(B.a[t], B.c[t]), Xb);
+            const Vec<LPL> U = plus(theta_of(B.a[t]), Xb);
             const float u = lane_weight(B.u, t);
             const Vec<LPL> m = minconv<LPL, DC, PE>(U, u, lane_weight(B.w, t), PE ? lane_weight(B.w2, t) : 0.f,
                                                     L1v, L2v, a.delta, cc);
@@ -904,8 +944,8 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
         for (int t = 0; t < K; ++t)
           if (t < cnt)
             st_buf<LPL, FULL>(sW, t * strideB, lane, D,
-                              step_f(i0 + t, plus(R, bs[t]), B.a[t], theta_of(B.a[t], B.c[t]), lane_weight(B.w, t),
-                                     PE ? lane_weight(B.w2, t) : 0.f));
+                              step_f(i0 + t, plus(R, bs[t]), row_unaries(B.a[t], B.c[t]), theta_of(B.a[t]),
+                                     lane_weight(B.w, t), PE ? lane_weight(B.w2, t) : 0.f));
       };
       // Two segments per iteration, alternating between X and Y; the next
       // segment's loads are issued before the current segment's steps.
@@ -938,17 +978,24 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
       const float wnext = wlane_f(i0 + P), wnext2 = wlane_f2(i0 + P);
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        // A_{i+1} lives on as the next step's a_cur: copied out of the slot.
-        const Vec<LPL> an = fresh(ra[j]);
-        Vec<LPL> thn = theta_of(an, rc[j]);
-        Vec<LPL> RB = plus(R, rb[j]);
-        if constexpr (kCols) {  // (rows: thn is `an` itself)
+        // A_{i+1} lives on as the next step's a_cur: copied out of the slot
+        // (columns form it from C and B).
+        Vec<LPL> an, thn;
+        if constexpr (kCols) {
+          an = row_unaries(ra[j], rc[j]);
+          thn = theta_of(ra[j]);
+          pin(an);
           pin(thn);
-          pin(RB);
+        } else {
+          an = fresh(ra[j]);
+          thn = an;
         }
+        const Vec<LPL> ci = fresh(rc[j]);  // (rows: C_i)
+        Vec<LPL> RB = plus(R, rb[j]);
+        if constexpr (kCols) pin(RB);
         __builtin_amdgcn_sched_barrier(kSchedValu);
         load_f(i0 + j + P, rb[j], ra[j], rc[j]);
-        store_f(i0 + j, step_f(i0 + j, RB, an, thn, lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f));
+        store_f(i0 + j, step_f(i0 + j, RB, an, thn, lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f), ci);
       }
       wf = wnext;
       wf2 = wnext2;
@@ -956,15 +1003,20 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
 #pragma unroll
     for (int j = 0; j < P; ++j)
       if (i0 + j < nb)
-        store_f(i0 + j, step_f(i0 + j, plus(R, rb[j]), ra[j], theta_of(ra[j], rc[j]), lane_weight(wf, j),
-                               PE ? lane_weight(wf2, j) : 0.f));
+        store_f(i0 + j,
+                step_f(i0 + j, plus(R, rb[j]), kCols ? row_unaries(ra[j], rc[j]) : ra[j], theta_of(ra[j]),
+                       lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f),
+                rc[j]);
     }
     {
       Vec<LPL> out;
 #pragma unroll
       for (int k = 0; k < LPL; ++k)
         out.v[k] = MODE == kRowsMinorant ? a_cur.v[k] - R.v[k] : a_cur.v[k] + R.v[k];
-      st_node<LPL, FULL>(Aw + (n - 1) * nstride, lane, D, out);
+      if constexpr (kCols)
+        st_node<LPL, FULL>(Aw + (n - 1) * nstride, lane, D, out);
+      else
+        st_node<LPL, FULL>(Aw + (n - 1) * nstride, lane, D, col_unaries(out, c_last));
     }
     const double kept = chain_kept(fw_lds, b_lds, n, lane, kGammaD);
     if (lane == 0) {

@@ -11,8 +11,8 @@ constexpr double kGammaD = 0.25;
 enum ChainMode { kRowsMinorant = 0, kColsMinorant = 1, kRowsDecode = 2 };
 
 struct ChainArgs {
-  const float* cost;   // C (column mode)
-  float* A;            // row unaries, updated in place
+  const float* cost;   // C (row and column sweeps)
+  float* A;            // state: row unaries before a row sweep, column unaries C - A before a column sweep
   const float* A_in;   // where a row sweep reads the unaries (A, or cost for the first sweep)
   const float* w;      // edge weight of the edge (node i, node i+1), indexed by pixel (P1 with w2)
   const float* w2;     // per-edge penalties: P2 of the edge (then w holds P1); nullptr: w L1, w L2

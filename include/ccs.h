@@ -268,18 +268,23 @@ int ccs_slackprop_pw(ccs_handle_t h, const float* cost, const float* pen, int le
  * (stereo/error_util.h:23-52 covers CUDA and cuDNN status codes only). */
 int ccs_slackprop_status(ccs_handle_t h);
 
-/* One half-step of ccs_slackprop on caller-held row unaries A (H x W x D, set to
- * a copy of cost before the first call): direction 0 sweeps the rows
- * (A -= minorant), 1 the columns (A += minorant, reads cost), 2 decodes the rows
- * into labels; 3 is the first row sweep of a run without that copy (it reads the
- * unaries from cost and writes A = cost - minorant, so A needs no initialisation).  chain_sum / kept_sum (optional device doubles) receive S_j / K_j.
+/* One half-step of ccs_slackprop on a caller-held state volume A (H x W x D)
+ * that alternates between the two subproblems' unaries, so that each sweep
+ * reads one volume for its chains: direction 0 sweeps the rows of the row
+ * unaries in A and leaves the column unaries there (A := cost - (A -
+ * minorant)); 1 sweeps the columns of those and leaves the row unaries
+ * (A := (cost - A) + minorant); 2 decodes the rows of row unaries into labels;
+ * 3 is the first row sweep of a run, which reads the row unaries from cost
+ * (A needs no initialisation).  A run is 3, then 1, then (0, 1) per further
+ * iteration, then 2.  Every sweep but 2 reads cost, which must not be A.
+ * chain_sum / kept_sum (optional device doubles) receive S_j / K_j.
  * The steps of slack_prop_2D_alg::execute (stereo/stereo.cpp:153), exposed for
  * callers that schedule or time the sweeps themselves. */
 int ccs_slackprop_sweep(ccs_handle_t h, const float* cost, float* A, const float* wx, const float* wy,
                         int H, int W, int D, float L1, float L2, int delta, int direction,
                         int* labels, double* chain_sum, double* kept_sum);
 
-/* Row decode of caller-held unaries (as left by ccs_slackprop_sweep) with the
+/* Row decode of caller-held row unaries (as a column sweep leaves them) with the
  * min-marginal gap: labels (H x W int32) = lowest argmin of the row min-marginals,
  * gap (H x W float32) = second-best minus best min-marginal, >= 0.  The gap marks
  * pixels whose label is decided by a near tie (parity checks use it); the decode
