@@ -1301,16 +1301,24 @@ static int cu_count() {
   return cus[dev];
 }
 
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+// LDS of one row chain with the message wave: b, w, u, [w2], v floats rounded
+// to 16 bytes, then the message ring.
+template <int LPL, bool PE>
+static size_t row_seg_lds(int n) {
+  return sizeof(float) * ((((PE ? 6 : 5) * (size_t)n + 3) & ~(size_t)3) +
+                          (size_t)kRowRing * seg_len<LPL>() * 64 * LPL);
+}
+
 template <int LPL, int MODE, int P, bool FULL, int DC, bool SEG, bool NANCHK, bool PE>
 static void launch_sweep(ChainArgs& a, size_t lds, hipStream_t st) {
   constexpr bool kRowSeg = MODE != kColsMinorant && SEG;
   int cpw = 1;
   if constexpr (kRowSeg) {
-    // per chain: b, w, u, [w2], v floats rounded to 16 bytes, then the ring
-    lds = sizeof(float) * ((((PE ? 6 : 5) * (size_t)a.n + 3) & ~(size_t)3) +
-                           (size_t)kRowRing * seg_len<LPL>() * 64 * LPL);
+    lds = row_seg_lds<LPL, PE>(a.n);
     const int cus = cu_count();
-    if (a.n_chains > cus && a.n_chains <= 2 * cus) cpw = 2;  // one workgroup per CU
+    if (a.n_chains > cus && a.n_chains <= 2 * cus && 2 * lds <= kLdsPerCu) cpw = 2;  // one workgroup per CU
     lds *= cpw;
   }
   if (lds > 64 * 1024)  // long chains: allow the LDS beyond the default 64 KB
@@ -1428,7 +1436,12 @@ int launch_chain(ChainMode mode, ChainArgs& a, hipStream_t st) {
   }
   // Row sweeps take the message wave at the same volumes (their chains are
   // addressed within the 2 GiB checked above).
-  const bool row_seg = kSeg > 0 && vol >= seg_min_bytes();
+  // The message wave also needs one chain's LDS to fit a CU.
+  const bool pe = a.w2 != nullptr;
+  const size_t row_lds = a.D <= 64    ? (pe ? row_seg_lds<1, true>(a.n) : row_seg_lds<1, false>(a.n))
+                         : a.D <= 128 ? (pe ? row_seg_lds<2, true>(a.n) : row_seg_lds<2, false>(a.n))
+                                      : (pe ? row_seg_lds<4, true>(a.n) : row_seg_lds<4, false>(a.n));
+  const bool row_seg = kSeg > 0 && vol >= seg_min_bytes() && row_lds <= kLdsPerCu;
   switch (mode) {
     case kRowsMinorant: return launch_mode<kRowsMinorant>(a, row_seg, st);
     case kColsMinorant: return launch_mode<kColsMinorant>(a, seg, st);
