@@ -854,17 +854,16 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
         }
         const int i = ip - g;
         const Vec<LPL> an = fresh(ra[j]);
-        const Vec<LPL> ci = fresh(rc[j]);
         const Vec<LPL> RB = plus(R, bm[j % 3]);
         __builtin_amdgcn_sched_barrier(kSchedValu);
-        if (!TAIL) {
-          const int off = __builtin_amdgcn_readlane(ao, j);
-          ra[j] = ld_buf<LPL, FULL>(rA, off, lane, D, kInfF);
-          rc[j] = ld_buf<LPL, FULL>(rC, off - strideB, lane, D, kInfF);
-        }
+        const int off = TAIL ? 0 : __builtin_amdgcn_readlane(ao, j);
+        if (!TAIL) ra[j] = ld_buf<LPL, FULL>(rA, off, lane, D, kInfF);
         bm[(j + 2) % 3] = ld_ring<LPL>(ring + ((j + 2) % P) * kNS, lane);
         if (!LOW || i >= 0)
-          store_f(i, step_f(i, RB, an, theta_of(an), lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f), ci);
+          store_f(i, step_f(i, RB, an, theta_of(an), lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f), rc[j]);
+        // C_i is read only by the store above: its slot is refilled after it,
+        // so the value needs no copy out of the ring.
+        if (!TAIL) rc[j] = ld_buf<LPL, FULL>(rC, off - strideB, lane, D, kInfF);
       };
       int i0p = 0;
       if (nbp >= P) {
