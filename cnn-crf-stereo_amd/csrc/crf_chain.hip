@@ -823,29 +823,37 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
       const int M = (nb + K - 1) / K;
       const int g = M * K - nb;
       const int nbp = M * K;
-      // A_{i+1} into ra, C_i into rc (the costs of the node that step stores).
-      auto load_a = [&](int ip, Vec<LPL>& av, Vec<LPL>& cv) {
-        const int node = max(min(ip - g, nb - 1), 0) + 1;
-        av = ld_buf<LPL, FULL>(rA, node * strideB, lane, D, kInfF);
-        cv = ld_buf<LPL, FULL>(rC, (node - 1) * strideB, lane, D, kInfF);
-      };
-      // Byte offsets of a block's refills (node of step ip0 + P + lane), one per
+      // Register ring slot j (steps ip = j mod P) holds A_{i+1}, read by step
+      // ip as theta_{i+1} and by step ip + 1 as its own node's unaries A_i; it
+      // is refilled after that second read, so neither read copies it out.
+      // Node of step ip's A (A_{i+1}, i = ip - g); virtual steps (i < 0) read
+      // A_0, which the first real step reads as its own node's unaries.
+      auto node_a = [&](int ip) { return max(min(ip - g, nb - 1) + 1, 0); };
+      // rc slot j holds C_i of the node step ip stores, refilled after the store.
+      auto node_c = [&](int ip) { return max(min(ip - g, nb - 1), 0); };
+      // Byte offsets of a block's refills (A of step ip0 + P + lane), one per
       // lane: a step takes its offset with v_readlane instead of clamping it.
-      auto refill_offsets = [&](int ip0) { return (max(min(ip0 + P - g + lane, nb - 1), 0) + 1) * strideB; };
+      auto refill_offsets = [&](int ip0) { return node_a(ip0 + P + lane) * strideB; };
       auto wl = [&](int ip0) { return w_lds[max(min(ip0 - g + lane, n - 2), 0)]; };
       auto wl2 = [&](int ip0) { return PE ? w2_lds[max(min(ip0 - g + lane, n - 2), 0)] : 0.f; };
       Vec<LPL> bm[3];
       if (nb > 0) {
+        // slots 0 .. P-2: steps 0 .. P-2; slot P-1: A of step -1 (A_0, the first
+        // step's own unaries), its C that of step P-1
 #pragma unroll
-        for (int j = 0; j < P; ++j) load_a(j, ra[j], rc[j]);
+        for (int j = 0; j < P; ++j) {
+          ra[j] = ld_buf<LPL, FULL>(rA, node_a(j < P - 1 ? j : -1) * strideB, lane, D, kInfF);
+          rc[j] = ld_buf<LPL, FULL>(rC, node_c(j) * strideB, lane, D, kInfF);
+        }
         lds_barrier();  // segments 0 and 1 are in the ring
         bm[0] = ld_ring<LPL>(ring, lane);
         bm[1] = ld_ring<LPL>(ring + kNS, lane);
       }
       float wf = wl(0), wf2 = wl2(0);
-      // Step j of the block at ip0.  LOW: the block may start with virtual
-      // steps (i < 0, skipped); TAIL: no refills, steps past the chain skipped.
-      auto fstep = [&](int ip0, int ao, int j, bool LOW, bool TAIL) {
+      // Step j of the block at ip0 (ao: the block's refill offsets, aop: the
+      // previous block's).  LOW: the block may start with virtual steps (i < 0,
+      // skipped); TAIL: no refills, steps past the chain skipped.
+      auto fstep = [&](int ip0, int ao, int aop, int j, bool LOW, bool TAIL) {
         const int ip = ip0 + j;
         if (TAIL && ip >= nbp) return;
         if (j % K == 0) {
@@ -853,25 +861,31 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
           if (q >= 1 && q <= M - 2) lds_barrier();
         }
         const int i = ip - g;
-        const Vec<LPL> an = fresh(ra[j]);
+        const int jp = (j + P - 1) % P;  // the previous step's slot
+        a_cur = ra[jp];                  // A_i
         const Vec<LPL> RB = plus(R, bm[j % 3]);
         __builtin_amdgcn_sched_barrier(kSchedValu);
-        const int off = TAIL ? 0 : __builtin_amdgcn_readlane(ao, j);
-        if (!TAIL) ra[j] = ld_buf<LPL, FULL>(rA, off, lane, D, kInfF);
         bm[(j + 2) % 3] = ld_ring<LPL>(ring + ((j + 2) % P) * kNS, lane);
         if (!LOW || i >= 0)
-          store_f(i, step_f(i, RB, an, theta_of(an), lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f), rc[j]);
-        // C_i is read only by the store above: its slot is refilled after it,
-        // so the value needs no copy out of the ring.
-        if (!TAIL) rc[j] = ld_buf<LPL, FULL>(rC, off - strideB, lane, D, kInfF);
+          store_f(i, step_f(i, RB, ra[j], theta_of(ra[j]), lane_weight(wf, j), PE ? lane_weight(wf2, j) : 0.f),
+                  rc[j]);
+        if (!TAIL) {
+          // C_i and A_i are read: their slots take C of step ip + P and A of
+          // step ip - 1 + P.
+          rc[j] = ld_buf<LPL, FULL>(rC, __builtin_amdgcn_readlane(ao, j) - strideB, lane, D, kInfF);
+          ra[jp] = ld_buf<LPL, FULL>(rA, j > 0 ? __builtin_amdgcn_readlane(ao, j - 1) : __builtin_amdgcn_readlane(aop, P - 1),
+                                     lane, D, kInfF);
+        }
       };
       int i0p = 0;
       if (nbp >= P) {
+        int aop = refill_offsets(-P);
         {
           const float wn = wl(P), wn2 = wl2(P);
           const int ao = refill_offsets(0);
 #pragma unroll
-          for (int j = 0; j < P; ++j) fstep(0, ao, j, true, false);
+          for (int j = 0; j < P; ++j) fstep(0, ao, aop, j, true, false);
+          aop = ao;
           wf = wn;
           wf2 = wn2;
         }
@@ -879,13 +893,15 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
           const float wn = wl(i0p + P), wn2 = wl2(i0p + P);
           const int ao = refill_offsets(i0p);
 #pragma unroll
-          for (int j = 0; j < P; ++j) fstep(i0p, ao, j, false, false);
+          for (int j = 0; j < P; ++j) fstep(i0p, ao, aop, j, false, false);
+          aop = ao;
           wf = wn;
           wf2 = wn2;
         }
       }
+      // (the last node's own unaries: a_cur as the last step left it)
 #pragma unroll
-      for (int j = 0; j < P; ++j) fstep(i0p, 0, j, true, true);
+      for (int j = 0; j < P; ++j) fstep(i0p, 0, 0, j, true, true);
     } else
     if constexpr (kCols && SEG && seg_len<LPL>() > 0) {
       // Segments of K forward steps i0 .. i0+K-1.  Their nodes i0+1 .. i0+K
