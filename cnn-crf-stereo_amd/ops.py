@@ -202,7 +202,9 @@ def slackprop(h, cost, wx, wy, L1, L2, delta, iters, want_unaries=False):
 def slackprop_sweep(h, cost, A, wx, wy, L1, L2, delta, direction, labels=None, chain_sum=None, kept_sum=None):
     """One half-step (0 rows, 1 columns, 2 decode, 3 the first row sweep of a run,
     which reads the unaries from cost so that A needs no copy of it) on the
-    caller's row unaries A."""
+    caller's state volume A: a row sweep reads the row unaries there and leaves
+    the column unaries cost - A', a column sweep the reverse; the decode reads
+    row unaries (as a column sweep leaves them)."""
     H, W, D = cost.shape
     call("ccs_slackprop_sweep", h.h, ptr(cost), ptr(A), ptr(wx), ptr(wy), H, W, D, float(L1), float(L2),
          int(delta), int(direction), ptr(labels), ptr(chain_sum), ptr(kept_sum))
