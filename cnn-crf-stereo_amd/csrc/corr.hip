@@ -472,7 +472,10 @@ extern "C" int ccs_correlation_volume(ccs_handle_t h, const void* f0, const void
   else if (ntm_need <= 9) ntm = 9;
   else if (ntm_need <= 17) ntm = 17;
   else return fail(CCS_ERR_UNSUPPORTED, "ccs_correlation_volume: disparity span %d exceeds 256", span);
-  a.sp = 16 * ntm;
+  // S row pitch: 16*ntm + 3 floats, so that band rows r start (sp+1) r = 20 r
+  // (mod 32) words apart and the softmax pass's four-word groups of eight rows
+  // fall on distinct banks (with a multiple of 16 the rows met every 2).
+  a.sp = 16 * ntm + 3;
   const size_t esz = precision == CCS_BF16 ? 2 : 4;
   a.pitch = precision == CCS_BF16 ? a.Cs : a.Cs + 1;
   // Waves per workgroup: the most resident waves per CU (160 KB of LDS), ties
