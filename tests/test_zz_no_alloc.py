@@ -65,6 +65,9 @@ def test_noalloc_equals_conv2d_forward(handle, prec, calls):
         assert torch.equal(got, want), (H, W, C, K, k)
 
 
+# The refused capture of the allocating entry point ends with nothing recorded,
+# which torch reports as a warning.
+@pytest.mark.filterwarnings("ignore:The CUDA Graph is empty")
 @pytest.mark.parametrize("prec", ["bf16", "fp32"])
 def test_noalloc_records_into_a_graph_on_a_fresh_handle(prec):
     rng = np.random.default_rng(7)
