@@ -599,7 +599,19 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
     if constexpr (kRowSeg) {
       constexpr int K = KR;
       const int M = (nb + K - 1) / K;
-      if (M == 0) return;
+      // NANCHK: this wave tests node 0 and every unary its segments consume
+      // (nodes 1 .. n-1) and sets the chain's flag, off the forward wave.
+      bool nan1 = NANCHK && any_nan(ld_buf<LPL, FULL>(rA, 0, lane, D, kInfF));
+      auto put_flag = [&]() {
+        if constexpr (NANCHK) {
+          const bool any = __ballot(nan1) != 0;  // all lanes take part
+          if (lane == 0) a.nan_flags[c] = any ? 1 : 0;
+        }
+      };
+      if (M == 0) {
+        put_flag();
+        return;
+      }
       const int g = M * K - nb;
       struct HSeg {
         Vec<LPL> a[K], m;
@@ -624,6 +636,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
 #pragma unroll
         for (int t = K - 1; t >= 0; --t) {
           if (ib + t >= 0) {
+            if constexpr (NANCHK) nan1 |= any_nan(B.a[t]);
             const Vec<LPL> U = plus(theta_of(B.a[t]), Xb);
             const float u = lane_weight(B.u, t);
             const Vec<LPL> m = minconv<LPL, DC, PE>(U, u, lane_weight(B.w, t), PE ? lane_weight(B.w2, t) : 0.f,
@@ -652,6 +665,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
         lds_barrier();
         ++s;
       }
+      put_flag();
     }
   };
   if (!kRowSeg || wid == 0) {
@@ -704,7 +718,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
     const int bo = kRowSeg ? (s0 + P + lane < nb ? n - 1 - (s0 + P + lane) : 1) * strideB : 0;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
+      if constexpr (NANCHK && !kRowSeg) nan_seen |= any_nan(ra[j]);
       // Slot j is consumed (U is a new value) before it is refilled, so the
       // refill reuses the slot's registers.
       const Vec<LPL> U = plus_bh(theta_of(ra[j]));
@@ -721,7 +735,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
 #pragma unroll
   for (int j = 0; j < P; ++j)
     if (s0 + j < nb) {
-      if constexpr (NANCHK) nan_seen |= any_nan(ra[j]);
+      if constexpr (NANCHK && !kRowSeg) nan_seen |= any_nan(ra[j]);
       step_b(s0 + j, plus_bh(theta_of(ra[j])), lane_weight(wv, j), PE ? lane_weight(wv2, j) : 0.f,
              (j + 1) % KR == 0);
     }
@@ -729,7 +743,7 @@ __global__ __launch_bounds__(MODE != kColsMinorant && SEG ? 256 : 64) void k_cha
 
   // bh holds Bh[0]; theta[0] is read directly.
   a0 = ld_node<LPL, FULL>(Ab, lane, D, kInfF);
-  if constexpr (NANCHK) {
+  if constexpr (NANCHK && !kRowSeg) {  // (kRowSeg: the message wave tests the unaries)
     nan_seen |= any_nan(a0);
     const bool any = __ballot(nan_seen) != 0;  // all lanes take part
     if (lane == 0) a.nan_flags[c] = any ? 1 : 0;

@@ -4,7 +4,8 @@
   stereo/operators/slackprop.h:46-47, slackprop.cpp:142-173) against the
   oracle's final row unaries and bound;
 * NaN costs reported as CCS_ERR_NAN by the synchronising statics and by
-  ccs_slackprop_status after the stream-ordered ccs_slackprop;
+  ccs_slackprop_status after the stream-ordered ccs_slackprop, also where the
+  row sweeps' message wave does the test (volumes of 64 MiB and more);
 * scratch growth refused while the handle's stream is being captured, and a
   capture after ccs_reserve replaying to the eager result;
 * the structured-SVM hinge with unlabelled pixels (targets -1) against the
@@ -73,6 +74,32 @@ def test_nan_cost_is_reported(handle):
     Ci = C.copy()
     Ci[2, 3, :7] = np.inf
     ops.slackprop(handle, dev(Ci), dev(wx), dev(wy), 0.5, 2.0, 2, 2)
+    ops.slackprop_status(handle)
+
+
+@pytest.mark.parametrize("H,x,y", [(140, 0, 37), (140, 1, 38), (140, 617, 99), (140, 1240, 0), (140, 1241, 139),
+                                   (375, 0, 200), (375, 1241, 201)])
+def test_nan_cost_is_reported_by_the_checkpointed_row_sweep(handle, H, x, y):
+    """Volumes of 64 MiB and more run the row sweeps with a message wave, which
+    tests the first sweep's unaries (node 0 and every node its segments read).
+    A NaN at a row's first, second, middle and last nodes is reported with its
+    row, with one chain per workgroup (140 rows) and two (375 rows), and a
+    clean volume clears the status."""
+    W, D = 1242, 128
+    g = torch.Generator(device="cuda")
+    g.manual_seed(77)
+    cost = -torch.softmax(torch.randn(H, W, D, device="cuda", generator=g) * 3, dim=-1)
+    wx = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    wy = torch.rand(H, W, device="cuda", generator=g) * 0.8 + 0.2
+    assert cost.numel() * 4 >= 64 << 20
+    d = 5 + x % 100
+    cost[y, x, d] = float("nan")
+    ops.slackprop(handle, cost, wx, wy, 0.5, 2.0, 2, 1)
+    with pytest.raises(capi.CCSError) as e:
+        ops.slackprop_status(handle)
+    assert e.value.status == capi.ERR_NAN and "in 1 row(s), the first at y = %d" % y in str(e.value)
+    cost[y, x, d] = -0.5
+    ops.slackprop(handle, cost, wx, wy, 0.5, 2.0, 2, 1)
     ops.slackprop_status(handle)
 
 
